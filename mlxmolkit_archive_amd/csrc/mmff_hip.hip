@@ -1,0 +1,1735 @@
+// Batched MMFF94 conformer minimization for gfx950 (MI355X): kernels + extern "C" ABI.
+//
+// Execution model (one conformer per workgroup, persistent grid with a work queue):
+//   * positions, gradient accumulators and BFGS/L-BFGS vectors live in LDS;
+//   * the dense inverse Hessian lives in LDS when it fits (n = 3N up to ~195 at one
+//     workgroup per CU), otherwise in a per-workgroup global slot (L2/MALL resident),
+//     stored column-per-thread so every H pass is conflict-free / coalesced;
+//   * term energies and gradients are FP32; each lane's energy is accumulated in FP64 and
+//     reduced with wave shuffles + one LDS exchange, so the Armijo test and reported
+//     energies are FP64 sums; gradients are summed exactly in 64-bit fixed point with LDS
+//     atomics (order-independent, bit-reproducible).
+// The control flow of each optimizer restates the reference kernel it replaces; the
+// file:line citations are on each kernel.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+
+#include "mmff_hip.h"
+#include "mmff_terms.h"
+
+using namespace mmff;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                  \
+  do {                                                                                 \
+    hipError_t _e = (expr);                                                            \
+    if (_e != hipSuccess)                                                              \
+      return fail(MMFFHIP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+// ------------------------------------------------------------------------------------
+// device-side view of the batch
+struct KBatch {
+  const int32_t* mol_desc;
+  const uint32_t* bond_ij;
+  const float2* bond_p;
+  const uint2* ang_idx;
+  const float2* ang_p;
+  const uint2* sb_idx;
+  const float4* sb_p;
+  const float* sb_t0;
+  const uint2* oop_idx;
+  const float* oop_k;
+  const uint2* tor_idx;
+  const float4* tor_p;
+  const uint32_t* pair_ij;
+  const float4* pair_p;
+  const int32_t* conf_mol;
+  const int32_t* conf_off;
+  int n_mols, n_confs, max_atoms;
+};
+
+KBatch to_kbatch(const mmffhip_batch* b) {
+  KBatch k;
+  k.mol_desc = b->mol_desc;
+  k.bond_ij = b->bond_ij;
+  k.bond_p = (const float2*)b->bond_p;
+  k.ang_idx = (const uint2*)b->ang_idx;
+  k.ang_p = (const float2*)b->ang_p;
+  k.sb_idx = (const uint2*)b->sb_idx;
+  k.sb_p = (const float4*)b->sb_p;
+  k.sb_t0 = b->sb_t0;
+  k.oop_idx = (const uint2*)b->oop_idx;
+  k.oop_k = b->oop_k;
+  k.tor_idx = (const uint2*)b->tor_idx;
+  k.tor_p = (const float4*)b->tor_p;
+  k.pair_ij = b->pair_ij;
+  k.pair_p = (const float4*)b->pair_p;
+  k.conf_mol = b->conf_mol;
+  k.conf_off = b->conf_off;
+  k.n_mols = b->n_mols;
+  k.n_confs = b->n_confs;
+  k.max_atoms = b->max_atoms;
+  return k;
+}
+
+struct MolDesc {
+  int n_atoms, bond_off, nb, ang_off, na, sb_off, nsb, oop_off, noop, tor_off, ntor, pair_off, npair;
+};
+
+__device__ __forceinline__ MolDesc load_desc(const int32_t* md) {
+  MolDesc d;
+  d.n_atoms = md[0];
+  d.bond_off = md[1];
+  d.nb = md[2];
+  d.ang_off = md[3];
+  d.na = md[4];
+  d.sb_off = md[5];
+  d.nsb = md[6];
+  d.oop_off = md[7];
+  d.noop = md[8];
+  d.tor_off = md[9];
+  d.ntor = md[10];
+  d.pair_off = md[11];
+  d.npair = md[12];
+  return d;
+}
+
+// ------------------------------------------------------------------------------------
+// 64-bit fixed-point gradient accumulation in LDS (scale 2^32). The split through FP64
+// is exact for |v| < 2^31, so the accumulated sum equals the exact sum of the FP32
+// contributions truncated to 2^-32, independent of arrival order.
+__device__ __forceinline__ unsigned long long to_fx(float v) {
+  double d = (double)v;
+  if (!(d == d)) d = 0.0;
+  d = fmin(fmax(d, -2147483000.0), 2147483000.0);
+  double h = floor(d);
+  unsigned int lo = (unsigned int)((d - h) * 4294967296.0);
+  int hi = (int)h;
+  return ((unsigned long long)(unsigned int)hi << 32) | (unsigned long long)lo;
+}
+
+__device__ __forceinline__ float from_fx(unsigned long long a) {
+  int hi = (int)(a >> 32);
+  unsigned int lo = (unsigned int)(a & 0xffffffffull);
+  return (float)((double)hi + (double)lo * 2.3283064365386962890625e-10);
+}
+
+template <bool G>
+struct LdsAcc {
+  unsigned long long* g;
+  __device__ __forceinline__ void add(int a, float x, float y, float z) {
+    if (G) {
+      atomicAdd(&g[3 * a], to_fx(x));
+      atomicAdd(&g[3 * a + 1], to_fx(y));
+      atomicAdd(&g[3 * a + 2], to_fx(z));
+    }
+  }
+};
+
+// ------------------------------------------------------------------------------------
+// energy (+ gradient) of one conformer whose positions are in LDS, spread over NT lanes.
+// Term tables are read from global memory (L2-resident: shared by the molecule's
+// conformers); each lane prefetches its next term's indices and parameters before it
+// evaluates the current one, so table latency overlaps the term arithmetic.
+// Returns this lane's FP64 energy partial; per-term partials in et[7] when PT.
+struct TermData {
+  uint32_t a, b;  // packed atom indices
+  float4 p;       // parameters
+  float q;        // fifth parameter (stretch-bend theta0)
+};
+
+// Pointers to one molecule's term tables (offsets folded in). Built either over the
+// global batch tables or over a copy staged in LDS.
+struct TView {
+  const uint32_t* bond_ij;
+  const float2* bond_p;
+  const uint2* ang_idx;
+  const float2* ang_p;
+  const uint2* sb_idx;
+  const float4* sb_p;
+  const float* sb_t0;
+  const uint2* oop_idx;
+  const float* oop_k;
+  const uint2* tor_idx;
+  const float4* tor_p;
+  const uint32_t* pair_ij;
+  const float4* pair_p;
+};
+
+__device__ __forceinline__ TView global_view(const KBatch& B, const MolDesc& m) {
+  TView v;
+  v.bond_ij = B.bond_ij + m.bond_off;
+  v.bond_p = B.bond_p + m.bond_off;
+  v.ang_idx = B.ang_idx + m.ang_off;
+  v.ang_p = B.ang_p + m.ang_off;
+  v.sb_idx = B.sb_idx + m.sb_off;
+  v.sb_p = B.sb_p + m.sb_off;
+  v.sb_t0 = B.sb_t0 + m.sb_off;
+  v.oop_idx = B.oop_idx + m.oop_off;
+  v.oop_k = B.oop_k + m.oop_off;
+  v.tor_idx = B.tor_idx + m.tor_off;
+  v.tor_p = B.tor_p + m.tor_off;
+  v.pair_ij = B.pair_ij + m.pair_off;
+  v.pair_p = B.pair_p + m.pair_off;
+  return v;
+}
+
+// bytes of one molecule's tables in the LDS staging layout
+__host__ __device__ inline size_t table_lds_bytes(int nb, int na, int nsb, int noop, int ntor, int npair) {
+  return 16 * (size_t)(ntor + nsb + npair) + 8 * (size_t)(2 * na + nsb + noop + ntor + nb) +
+         4 * (size_t)(nb + nsb + noop + npair);
+}
+
+template <class T>
+__device__ __forceinline__ void stage_copy(T* dst, const T* src, int count, int tid, int nt) {
+  const int words = count * (int)(sizeof(T) / 4);
+  uint32_t* d = (uint32_t*)dst;
+  const uint32_t* s = (const uint32_t*)src;
+  for (int i = tid; i < words; i += nt) d[i] = s[i];
+}
+
+// Copy a molecule's tables into LDS at `base` (16-byte aligned) and return a view of them.
+__device__ __forceinline__ TView stage_tables(const KBatch& B, const MolDesc& m, unsigned char* base, int tid,
+                                              int nt) {
+  TView v;
+  unsigned char* p = base;
+  auto take = [&](size_t bytes) {
+    unsigned char* q = p;
+    p += bytes;
+    return q;
+  };
+  float4* tor_p = (float4*)take(16 * (size_t)m.ntor);
+  float4* sb_p = (float4*)take(16 * (size_t)m.nsb);
+  float4* pair_p = (float4*)take(16 * (size_t)m.npair);
+  uint2* ang_idx = (uint2*)take(8 * (size_t)m.na);
+  float2* ang_p = (float2*)take(8 * (size_t)m.na);
+  uint2* sb_idx = (uint2*)take(8 * (size_t)m.nsb);
+  uint2* oop_idx = (uint2*)take(8 * (size_t)m.noop);
+  uint2* tor_idx = (uint2*)take(8 * (size_t)m.ntor);
+  float2* bond_p = (float2*)take(8 * (size_t)m.nb);
+  uint32_t* bond_ij = (uint32_t*)take(4 * (size_t)m.nb);
+  float* sb_t0 = (float*)take(4 * (size_t)m.nsb);
+  float* oop_k = (float*)take(4 * (size_t)m.noop);
+  uint32_t* pair_ij = (uint32_t*)take(4 * (size_t)m.npair);
+  stage_copy(tor_p, B.tor_p + m.tor_off, m.ntor, tid, nt);
+  stage_copy(sb_p, B.sb_p + m.sb_off, m.nsb, tid, nt);
+  stage_copy(pair_p, B.pair_p + m.pair_off, m.npair, tid, nt);
+  stage_copy(ang_idx, B.ang_idx + m.ang_off, m.na, tid, nt);
+  stage_copy(ang_p, B.ang_p + m.ang_off, m.na, tid, nt);
+  stage_copy(sb_idx, B.sb_idx + m.sb_off, m.nsb, tid, nt);
+  stage_copy(oop_idx, B.oop_idx + m.oop_off, m.noop, tid, nt);
+  stage_copy(tor_idx, B.tor_idx + m.tor_off, m.ntor, tid, nt);
+  stage_copy(bond_p, B.bond_p + m.bond_off, m.nb, tid, nt);
+  stage_copy(bond_ij, B.bond_ij + m.bond_off, m.nb, tid, nt);
+  stage_copy(sb_t0, B.sb_t0 + m.sb_off, m.nsb, tid, nt);
+  stage_copy(oop_k, B.oop_k + m.oop_off, m.noop, tid, nt);
+  stage_copy(pair_ij, B.pair_ij + m.pair_off, m.npair, tid, nt);
+  v.bond_ij = bond_ij;
+  v.bond_p = bond_p;
+  v.ang_idx = ang_idx;
+  v.ang_p = ang_p;
+  v.sb_idx = sb_idx;
+  v.sb_p = sb_p;
+  v.sb_t0 = sb_t0;
+  v.oop_idx = oop_idx;
+  v.oop_k = oop_k;
+  v.tor_idx = tor_idx;
+  v.tor_p = tor_p;
+  v.pair_ij = pair_ij;
+  v.pair_p = pair_p;
+  return v;
+}
+
+__device__ __forceinline__ TermData load_term(const TView& V, int t, int c1, int c2, int c3, int c4, int c5) {
+  TermData d;
+  d.q = 0.f;
+  if (t < c1) {
+    d.a = V.bond_ij[t];
+    d.b = 0;
+    const float2 q = V.bond_p[t];
+    d.p = make_float4(q.x, q.y, 0.f, 0.f);
+  } else if (t < c2) {
+    const int u = t - c1;
+    const uint2 ix = V.ang_idx[u];
+    d.a = ix.x;
+    d.b = ix.y;
+    const float2 q = V.ang_p[u];
+    d.p = make_float4(q.x, q.y, 0.f, 0.f);
+  } else if (t < c3) {
+    const int u = t - c2;
+    const uint2 ix = V.sb_idx[u];
+    d.a = ix.x;
+    d.b = ix.y;
+    d.p = V.sb_p[u];
+    d.q = V.sb_t0[u];
+  } else if (t < c4) {
+    const int u = t - c3;
+    const uint2 ix = V.oop_idx[u];
+    d.a = ix.x;
+    d.b = ix.y;
+    d.p = make_float4(V.oop_k[u], 0.f, 0.f, 0.f);
+  } else if (t < c5) {
+    const int u = t - c4;
+    const uint2 ix = V.tor_idx[u];
+    d.a = ix.x;
+    d.b = ix.y;
+    d.p = V.tor_p[u];
+  } else {
+    const int u = t - c5;
+    d.a = V.pair_ij[u];
+    d.b = 0;
+    d.p = V.pair_p[u];
+  }
+  return d;
+}
+
+template <int FAM, bool G, bool PT, int NT>
+__device__ __forceinline__ double eval_terms(const TView& TV, const MolDesc& m, const float* xs,
+                                             unsigned long long* gacc, int diel, uint32_t mask, double* et) {
+  LdsAcc<G> acc{gacc};
+  double e = 0.0;
+  // bonded terms (one generic loop, next term prefetched)
+  const int c1 = m.nb, c2 = c1 + m.na, c3 = c2 + m.nsb, c4 = c3 + m.noop, c5 = c4 + m.ntor;
+  int t = threadIdx.x;
+  TermData cur;
+  if (t < c5) cur = load_term(TV, t, c1, c2, c3, c4, c5);
+  while (t < c5) {
+    const int tn = t + NT;
+    TermData nxt;
+    if (tn < c5) nxt = load_term(TV, tn, c1, c2, c3, c4, c5);
+    const int i0 = (int)(cur.a & 0xffff), i1 = (int)(cur.a >> 16);
+    const int i2 = (int)(cur.b & 0xffff), i3 = (int)(cur.b >> 16);
+    if (t < c1) {
+      if (mask & 1u) {
+        const float v = t_bond<FAM, G>(xs, acc, i0, i1, cur.p.x, cur.p.y);
+        e += v;
+        if (PT) et[0] += v;
+      }
+    } else if (t < c2) {
+      if (mask & 2u) {
+        const float v = t_angle<FAM, G>(xs, acc, i0, i1, i2, cur.p.x, cur.p.y, i3 != 0);
+        e += v;
+        if (PT) et[1] += v;
+      }
+    } else if (t < c3) {
+      if (mask & 4u) {
+        const float v = t_strbend<FAM, G>(xs, acc, i0, i1, (int)cur.b, cur.p.x, cur.p.y, cur.p.z, cur.p.w, cur.q);
+        e += v;
+        if (PT) et[2] += v;
+      }
+    } else if (t < c4) {
+      if (mask & 8u) {
+        const float v = t_oop<FAM, G>(xs, acc, i0, i1, i2, i3, cur.p.x);
+        e += v;
+        if (PT) et[3] += v;
+      }
+    } else {
+      if (mask & 16u) {
+        const float v = t_torsion<FAM, G>(xs, acc, i0, i1, i2, i3, cur.p.x, cur.p.y, cur.p.z);
+        e += v;
+        if (PT) et[4] += v;
+      }
+    }
+    cur = nxt;
+    t = tn;
+  }
+  // non-bonded pairs: two pairs per lane per step (independent dependency chains),
+  // the next two prefetched
+  if (mask & 96u) {
+    const float ke = (mask & 32u) ? 1.0f : 0.0f, kq = (mask & 64u) ? 1.0f : 0.0f;
+    const int np = m.npair;
+    int u = threadIdx.x;
+    uint32_t ija = 0, ijb = 0;
+    float4 pa = make_float4(1.f, 0.f, 0.f, 0.f), pb = pa;
+    if (u < np) {
+      ija = TV.pair_ij[u];
+      pa = TV.pair_p[u];
+    }
+    if (u + NT < np) {
+      ijb = TV.pair_ij[u + NT];
+      pb = TV.pair_p[u + NT];
+    }
+    double ev_acc = 0.0, ee_acc = 0.0;
+    while (u < np) {
+      const int un = u + 2 * NT;
+      uint32_t nija = 0, nijb = 0;
+      float4 npa = make_float4(1.f, 0.f, 0.f, 0.f), npb = npa;
+      if (un < np) {
+        nija = TV.pair_ij[un];
+        npa = TV.pair_p[un];
+      }
+      if (un + NT < np) {
+        nijb = TV.pair_ij[un + NT];
+        npb = TV.pair_p[un + NT];
+      }
+      float eva, eea;
+      t_pair<FAM, G>(xs, acc, (int)(ija & 0xffff), (int)(ija >> 16), pa.x, ke * pa.y, kq * pa.z, diel, eva, eea);
+      ev_acc += eva;
+      ee_acc += eea;
+      if (u + NT < np) {
+        float evb, eeb;
+        t_pair<FAM, G>(xs, acc, (int)(ijb & 0xffff), (int)(ijb >> 16), pb.x, ke * pb.y, kq * pb.z, diel, evb, eeb);
+        ev_acc += evb;
+        ee_acc += eeb;
+      }
+      ija = nija;
+      ijb = nijb;
+      pa = npa;
+      pb = npb;
+      u = un;
+    }
+    e += ev_acc + ee_acc;
+    if (PT) {
+      et[5] += ev_acc;
+      et[6] += ee_acc;
+    }
+  }
+  return e;
+}
+
+// ------------------------------------------------------------------------------------
+// block reductions: wave shuffles, then one LDS exchange combined in fixed wave order
+// Wave-level FP64 reductions with DPP: a butterfly inside each 16-lane row
+// (quad xor 1, quad xor 2, half-row mirror, row mirror), then the four row results are
+// read with readlane and combined in a fixed order, so every lane gets the same value.
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_d(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+constexpr int DPP_QUAD_XOR1 = 0xB1, DPP_QUAD_XOR2 = 0x4E, DPP_ROW_HALF_MIRROR = 0x141, DPP_ROW_MIRROR = 0x140;
+__device__ __forceinline__ double wsum(double v) {
+  v += dpp_d<DPP_QUAD_XOR1>(v);
+  v += dpp_d<DPP_QUAD_XOR2>(v);
+  v += dpp_d<DPP_ROW_HALF_MIRROR>(v);
+  v += dpp_d<DPP_ROW_MIRROR>(v);
+  return ((lane_d(v, 0) + lane_d(v, 16)) + lane_d(v, 32)) + lane_d(v, 48);
+}
+__device__ __forceinline__ double wmax(double v) {
+  v = fmax(v, dpp_d<DPP_QUAD_XOR1>(v));
+  v = fmax(v, dpp_d<DPP_QUAD_XOR2>(v));
+  v = fmax(v, dpp_d<DPP_ROW_HALF_MIRROR>(v));
+  v = fmax(v, dpp_d<DPP_ROW_MIRROR>(v));
+  return fmax(fmax(lane_d(v, 0), lane_d(v, 16)), fmax(lane_d(v, 32), lane_d(v, 48)));
+}
+
+template <int NT>
+struct Reducer {
+  static constexpr int NW = NT / 64;
+  double* buf;  // RED_DOUBLES(NT) = 2 phases * NW * 8 doubles
+  int phase;
+  // K values; bit k of MAXMASK selects max instead of sum
+  template <int K, int MAXMASK>
+  __device__ __forceinline__ void run(double (&v)[K]) {
+#pragma unroll
+    for (int k = 0; k < K; k++) v[k] = ((MAXMASK >> k) & 1) ? wmax(v[k]) : wsum(v[k]);
+    static_assert(K <= 8, "at most 8 values per reduction");
+    double* b = buf + phase * (NW * 8);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < K; k++) b[w * 8 + k] = v[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+      double a = b[k];
+      for (int ww = 1; ww < NW; ww++) a = ((MAXMASK >> k) & 1) ? fmax(a, b[ww * 8 + k]) : a + b[ww * 8 + k];
+      v[k] = a;
+    }
+    phase ^= 1;
+  }
+  __device__ __forceinline__ double sum(double a) {
+    double v[1] = {a};
+    run<1, 0>(v);
+    return v[0];
+  }
+  __device__ __forceinline__ double max(double a) {
+    double v[1] = {a};
+    run<1, 1>(v);
+    return v[0];
+  }
+};
+
+// ------------------------------------------------------------------------------------
+// Kernel: single-point energy / gradient / per-term energies, one workgroup per conformer.
+// Replaces mmff_energy_grad_metal(_mega) (mmff_metal_kernel.py:713-775); the TG family is
+// what the nk kernel evaluates (mmff_bfgs_header.metal).
+template <int FAM, int NT>
+__global__ void __launch_bounds__(NT) k_energy_grad(KBatch B, const float* __restrict__ x, double* e_out,
+                                                    double* e_terms, float* g_out, uint32_t mask, int diel) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int c = blockIdx.x;
+  if (c >= B.n_confs) return;
+  constexpr int NW = NT / 64;
+  double* red = (double*)smem;
+  unsigned long long* gacc = (unsigned long long*)(red + 16 * NW);
+  const int mx = 3 * B.max_atoms;
+  float* xs = (float*)(gacc + mx);
+  const MolDesc m = load_desc(B.mol_desc + 16 * B.conf_mol[c]);
+  const int n = 3 * m.n_atoms;
+  const int off = B.conf_off[c];
+  for (int i = threadIdx.x; i < n; i += NT) {
+    xs[i] = x[off + i];
+    gacc[i] = 0ull;
+  }
+  __syncthreads();
+  Reducer<NT> R{red, 0};
+  double et[7] = {0, 0, 0, 0, 0, 0, 0};
+  double e;
+  if (g_out)
+    e = eval_terms<FAM, true, true, NT>(global_view(B, m), m, xs, gacc, diel, mask, et);
+  else
+    e = eval_terms<FAM, false, true, NT>(global_view(B, m), m, xs, gacc, diel, mask, et);
+  e = R.sum(e);  // includes the barrier that completes the gradient atomics
+  if (e_terms) {
+    double v4[4] = {et[0], et[1], et[2], et[3]};
+    R.template run<4, 0>(v4);
+    double v3[4] = {et[4], et[5], et[6], 0.0};
+    R.template run<4, 0>(v3);
+    if (threadIdx.x == 0) {
+      for (int k = 0; k < 4; k++) e_terms[7 * c + k] = v4[k];
+      for (int k = 0; k < 3; k++) e_terms[7 * c + 4 + k] = v3[k];
+    }
+  }
+  if (threadIdx.x == 0) e_out[c] = e;
+  if (g_out)
+    for (int i = threadIdx.x; i < n; i += NT) g_out[off + i] = from_fx(gacc[i]);
+}
+
+// ------------------------------------------------------------------------------------
+// nk minimizer (in-kernel BFGS / L-BFGS), TG formula family.
+// Control flow: mlxmolkit/mmff_bfgs_source_tg.metal:1-372 (BFGS) and
+// mlxmolkit/mmff_lbfgs_source_tg.metal:1-298 (L-BFGS); constants mmff_bfgs_header.metal:3-10.
+constexpr double TOLX = 1.2e-6;
+constexpr double FUNCTOL = 1e-4;
+constexpr double MOVETOL = 1e-6;
+constexpr double EPS_GUARD = 3e-7;
+constexpr double MAX_STEP_FACTOR = 100.0;
+constexpr int MAX_LS_ITERS = 1000;
+constexpr float GRAD_SCALE_INIT = 0.1f;
+constexpr double GRAD_CAP = 10.0;
+constexpr int LBFGS_MAX_M = 16;
+
+struct NkParams {
+  const float* x_in;
+  float* x_out;
+  double* e_out;
+  int32_t* status;
+  int32_t* counters;
+  const int32_t* order;
+  int32_t* queue;     // one int, zeroed before launch
+  float* hglob;       // per-workgroup global slots (H or L-BFGS history)
+  size_t slot_floats;
+  const float* precond;
+  int max_iters;
+  float grad_tol;
+  int diel;
+  int m;
+  int max_n;          // 3 * max_atoms rounded up to a multiple of 4
+  int ls_grad;
+  int tbytes;         // LDS bytes reserved for staged term tables (TLDS kernels)
+};
+
+__host__ __device__ inline size_t packed_floats(int n) { return (size_t)n * (n + 1) / 2; }
+
+// LDS footprint (bytes) of the nk kernel
+__host__ __device__ inline size_t nk_lds_bytes(int algo, bool hlds, int nt, int max_n, int m, size_t tbytes = 0) {
+  size_t nw = (size_t)nt / 64;
+  size_t b = 16 * nw * sizeof(double) + 64 + 2 * (size_t)LBFGS_MAX_M * sizeof(double);
+  b += (size_t)max_n * 8;       // gradient accumulators
+  b += (size_t)max_n * 4 * 4;   // xs, d, xo, hd
+  b += (size_t)max_n * 16;      // V = {dg, g, pending xi, pending hd}
+  b += (size_t)max_n * 8;       // per-lane coefficients of the pending update
+  if (algo == MMFFHIP_LBFGS) {
+    if (hlds) b += 2 * (size_t)m * max_n * 4;
+  } else if (hlds) {
+    b += ((packed_floats(max_n) * 4) + 15) & ~(size_t)15;
+  }
+  b += (tbytes + 15) & ~(size_t)15;
+  return b;
+}
+
+// initial inverse Hessian element (identity or the learned per-atom 3x3 block)
+__device__ __forceinline__ float h_init(const float* pc, int i, int j) {
+  if (pc && (i / 3) == (j / 3)) return pc[(j / 3) * 9 + (i % 3) * 3 + (j % 3)];
+  return i == j ? 1.0f : 0.0f;
+}
+
+template <int ALGO, bool HLDS, bool TLDS, int NT>
+__global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int NW = NT / 64;
+  const int tid = threadIdx.x;
+  const int mn = P.max_n;
+  double* red = (double*)smem;
+  int* s_int = (int*)(red + 16 * NW);
+  double* lb_sc = (double*)(s_int + 16);  // rho[16], alpha[16]
+  unsigned long long* gacc = (unsigned long long*)(lb_sc + 2 * LBFGS_MAX_M);
+  float4* V = (float4*)(gacc + mn);  // {dg, g, pending xi, pending hd}
+  float* xs = (float*)(V + mn);
+  float* d = xs + mn;   // direction (also y2 = H g scratch)
+  float* xo = d + mn;   // previous x, then xi = x - x_old
+  float* hd = xo + mn;  // H dg (BFGS) / two-loop q (L-BFGS)
+  float2* AB = (float2*)(hd + mn);  // pending update U(t, j) = AB[t].x xi_j + AB[t].y hd_j
+  float* big = (float*)(AB + mn);
+  float* H = HLDS ? big : P.hglob + (size_t)blockIdx.x * P.slot_floats;
+  size_t big_floats = 0;
+  if (HLDS) big_floats = ALGO == MMFFHIP_BFGS ? ((packed_floats(mn) + 3) & ~(size_t)3) : 2 * (size_t)P.m * mn;
+  unsigned char* tstage = (unsigned char*)(big + big_floats);
+  const int M = P.m;
+  float* S = H;
+  float* Y = H + (size_t)M * mn;
+  double* rho = lb_sc;
+  double* alf = lb_sc + LBFGS_MAX_M;
+
+  Reducer<NT> R{red, 0};
+
+  for (;;) {
+    if (tid == 0) s_int[0] = atomicAdd(P.queue, 1);
+    __syncthreads();
+    const int k = s_int[0];
+    __syncthreads();
+    if (k >= B.n_confs) break;
+    const int c = P.order ? P.order[k] : k;
+    const MolDesc m = load_desc(B.mol_desc + 16 * B.conf_mol[c]);
+    const int n = 3 * m.n_atoms;
+    const int off = B.conf_off[c];
+    const float* pc = P.precond ? P.precond + (size_t)(off / 3) * 9 : nullptr;
+    TView TV;
+    if (TLDS) {
+      if (table_lds_bytes(m.nb, m.na, m.nsb, m.noop, m.ntor, m.npair) > (size_t)P.tbytes) {
+        // tables larger than the reserved LDS: report and skip (the planner never does this)
+        for (int i = tid; i < n; i += NT) P.x_out[off + i] = P.x_in[off + i];
+        if (tid == 0) {
+          P.e_out[c] = __longlong_as_double(0x7ff8000000000000ll);
+          P.status[c] = 2;
+          if (P.counters) P.counters[3 * c] = P.counters[3 * c + 1] = P.counters[3 * c + 2] = 0;
+        }
+        continue;
+      }
+      TV = stage_tables(B, m, tstage, tid, NT);
+    } else {
+      TV = global_view(B, m);
+    }
+
+    for (int i = tid; i < n; i += NT) {
+      xs[i] = P.x_in[off + i];
+      gacc[i] = 0ull;
+    }
+    auto init_h = [&]() {
+      if (ALGO != MMFFHIP_BFGS) return;
+      if (HLDS) {
+        for (int col = tid; col < n; col += NT) {
+          float* cp = H + packed_floats(col);
+          for (int i = 0; i <= col; i++) cp[i] = h_init(pc, i, col);
+        }
+      } else {
+        for (int col = tid; col < n; col += NT)
+          for (int j = 0; j < n; j++) H[(size_t)j * n + col] = h_init(pc, j, col);
+      }
+    };
+    init_h();
+    __syncthreads();
+
+    int n_grads = 1, n_evals = 0, iters = 0;
+    // initial energy + gradient (mmff_bfgs_source_tg.metal:115-135)
+    double E = R.sum(eval_terms<FAM_TG, true, false, NT>(TV, m, xs, gacc, P.diel, 0x7fu, nullptr));
+    float gscale = GRAD_SCALE_INIT;
+    {
+      double mxg = 0.0;
+      for (int i = tid; i < n; i += NT) mxg = fmax(mxg, (double)fabsf(from_fx(gacc[i]) * gscale));
+      mxg = R.max(mxg);
+      while (mxg > GRAD_CAP) {
+        gscale *= 0.5f;
+        mxg *= 0.5;
+      }
+      for (int i = tid; i < n; i += NT) V[i] = make_float4(0.f, from_fx(gacc[i]) * gscale, 0.f, 0.f);
+    }
+    __syncthreads();
+    // initial direction: -g, or -H0 g with the learned preconditioner
+    for (int i = tid; i < n; i += NT) {
+      if (pc && ALGO == MMFFHIP_BFGS) {
+        const int a = i / 3;
+        float sacc = 0.f;
+        for (int q = 0; q < 3; q++) sacc += h_init(pc, i, 3 * a + q) * V[3 * a + q].y;
+        d[i] = -sacc;
+      } else {
+        d[i] = -V[i].y;
+      }
+    }
+    double ss = 0.0;
+    for (int i = tid; i < n; i += NT) ss += (double)xs[i] * xs[i];
+    ss = R.sum(ss);
+    const double max_step = MAX_STEP_FACTOR * fmax(sqrt(ss), (double)n);  // :141-145
+    int status = 1;
+    int hist_count = 0, hist_idx = 0;
+    bool fallback_used = (pc == nullptr);
+    bool pend = false;  // a rank-2 update is pending (applied lazily in the next H pass)
+
+    for (int it = 0; it < P.max_iters; it++) {
+      iters = it + 1;
+      const double E_old = E;
+      double v3[3] = {0.0, 0.0, 0.0};
+      for (int i = tid; i < n; i += NT) {
+        xo[i] = xs[i];
+        v3[0] += (double)d[i] * d[i];
+        v3[1] += (double)d[i] * V[i].y;
+        v3[2] = fmax(v3[2], (double)(fabsf(d[i]) / fmaxf(fabsf(xs[i]), 1.0f)));
+      }
+      R.template run<3, 4>(v3);
+      double slope = v3[1], test = v3[2];
+      const double dn = sqrt(v3[0]);
+      if (dn > max_step) {  // :157-166
+        const float sc = (float)(max_step / dn);
+        for (int i = tid; i < n; i += NT) d[i] *= sc;
+        slope *= (double)sc;
+        test *= (double)sc;
+      }
+      if (slope >= 0.0) {  // ascent bail-out :170-174
+        status = 0;
+        break;
+      }
+      const double lam_min = MOVETOL / fmax(test, 1e-30);
+      // backtracking line search :192-256
+      double lam = 1.0, prev_lam = 1.0, prev_e = E_old;
+      bool accepted = false, have_grad = false;
+      for (int ls = 0; ls < MAX_LS_ITERS; ls++) {
+        if (lam < lam_min) break;
+        const float lf = (float)lam;
+        for (int i = tid; i < n; i += NT) {
+          xs[i] = xo[i] + lf * d[i];
+          if (P.ls_grad) gacc[i] = 0ull;
+        }
+        __syncthreads();
+        double te;
+        if (P.ls_grad)
+          te = R.sum(eval_terms<FAM_TG, true, false, NT>(TV, m, xs, gacc, P.diel, 0x7fu, nullptr));
+        else
+          te = R.sum(eval_terms<FAM_TG, false, false, NT>(TV, m, xs, gacc, P.diel, 0x7fu, nullptr));
+        n_evals++;
+        if (te - E_old <= FUNCTOL * lam * slope) {
+          E = te;
+          accepted = true;
+          have_grad = P.ls_grad != 0;
+          break;
+        }
+        double tmp;
+        if (ls == 0) {
+          tmp = -slope / (2.0 * (te - E_old - slope));
+        } else {
+          const double rhs1 = te - E_old - lam * slope;
+          const double rhs2 = prev_e - E_old - prev_lam * slope;
+          const double l2 = lam * lam, pl2 = prev_lam * prev_lam, dv = lam - prev_lam;
+          if (fabs(dv) < 1e-30) {
+            tmp = 0.5 * lam;
+          } else {
+            const double a = (rhs1 / l2 - rhs2 / pl2) / dv;
+            const double b = (-prev_lam * rhs1 / l2 + lam * rhs2 / pl2) / dv;
+            if (fabs(a) < 1e-30) {
+              tmp = fabs(b) > 1e-30 ? -slope / (2.0 * b) : 0.5 * lam;
+            } else {
+              const double disc = b * b - 3.0 * a * slope;
+              if (disc < 0.0)
+                tmp = 0.5 * lam;
+              else if (b <= 0.0)
+                tmp = (-b + sqrt(disc)) / (3.0 * a);
+              else
+                tmp = -slope / (b + sqrt(disc));
+            }
+          }
+        }
+        tmp = fmin(tmp, 0.5 * lam);
+        tmp = fmax(tmp, 0.1 * lam);
+        prev_lam = lam;
+        prev_e = te;
+        lam = tmp;
+      }
+      if (!accepted) {
+        for (int i = tid; i < n; i += NT) xs[i] = xo[i];
+        if (!fallback_used) {
+          // learned preconditioner fallback: restart as identity-scaled BFGS
+          fallback_used = true;
+          pc = nullptr;
+          pend = false;
+          init_h();
+          for (int i = tid; i < n; i += NT) d[i] = -V[i].y;
+          __syncthreads();
+          continue;
+        }
+      }
+      // xi = x - x_old and TOLX :258-279
+      double tx = 0.0;
+      for (int i = tid; i < n; i += NT) {
+        const float xi = xs[i] - xo[i];
+        xo[i] = xi;
+        tx = fmax(tx, (double)(fabsf(xi) / fmaxf(fabsf(xs[i]), 1.0f)));
+      }
+      tx = R.max(tx);
+      if (tx < TOLX) {
+        status = 0;
+        break;
+      }
+      // new gradient :285-298
+      if (!have_grad) {
+        for (int i = tid; i < n; i += NT) gacc[i] = 0ull;
+        __syncthreads();
+        eval_terms<FAM_TG, true, false, NT>(TV, m, xs, gacc, P.diel, 0x7fu, nullptr);
+        __syncthreads();
+      }
+      n_grads++;
+      double v2[2] = {0.0, 0.0};
+      for (int i = tid; i < n; i += NT) {
+        const float sg = fabsf(from_fx(gacc[i]) * gscale);
+        v2[0] = fmax(v2[0], (double)sg);
+        v2[1] = fmax(v2[1], (double)sg * fmax((double)fabsf(xs[i]), 1.0));
+      }
+      R.template run<2, 3>(v2);
+      while (v2[0] > GRAD_CAP) {  // scale_grad_serial, mmff_bfgs_header.metal:435-454
+        gscale *= 0.5f;
+        v2[0] *= 0.5;
+        v2[1] *= 0.5;
+      }
+      for (int i = tid; i < n; i += NT) {
+        const float gn = from_fx(gacc[i]) * gscale;
+        float4 v = V[i];
+        v.x = gn - v.y;
+        v.y = gn;
+        V[i] = v;
+      }
+      if (v2[1] / fmax(E * (double)gscale, 1.0) < (double)P.grad_tol) {  // :300-316
+        status = 0;
+        break;
+      }
+      __syncthreads();  // V visible to every lane
+
+      if (ALGO == MMFFHIP_BFGS) {
+        // One pass over H computes hd = H_k dg and y2 = H_k g, where H_k = H_stored +
+        // pending rank-2 update; the pending update is written back in the same pass.
+        // The update U = fi s s^T - fei h h^T + fae a a^T (a = fi s - fei h) is kept in the
+        // factored form U(t, j) = alpha_t s_j + beta_t h_j (2 FMAs per element).
+        // Reference: H dg (:318-324), dots (:326-345), rank-2 update (:347-358), d = -H g (:360-366).
+        if (HLDS) {
+          // packed upper triangle, column j holds rows 0..j. Phase 1: lane t reads row t of
+          // the columns right of it; phase 2 (after a barrier): lane t owns column t and
+          // writes the updated values back. Loops are unrolled by 4 with independent
+          // accumulators so that four LDS reads are in flight per lane.
+          for (int t = tid; t < n; t += NT) {
+            const float2 ab = AB[t];
+            const float ux = pend ? ab.x : 0.f, uy = pend ? ab.y : 0.f;
+            float a1[4] = {0.f, 0.f, 0.f, 0.f}, a2[4] = {0.f, 0.f, 0.f, 0.f};
+            int j = t + 1;
+            size_t bj = packed_floats(j) + t;
+            for (; j + 3 < n; j += 4) {
+              const size_t b1 = bj + j + 1, b2 = b1 + j + 2, b3 = b2 + j + 3;
+              const float h0 = H[bj], h1 = H[b1], h2 = H[b2], h3 = H[b3];
+              const float4 v0 = V[j], v1 = V[j + 1], v2 = V[j + 2], v3 = V[j + 3];
+              const float g0 = h0 + ux * v0.z + uy * v0.w, g1 = h1 + ux * v1.z + uy * v1.w;
+              const float g2 = h2 + ux * v2.z + uy * v2.w, g3 = h3 + ux * v3.z + uy * v3.w;
+              a1[0] += g0 * v0.x;
+              a2[0] += g0 * v0.y;
+              a1[1] += g1 * v1.x;
+              a2[1] += g1 * v1.y;
+              a1[2] += g2 * v2.x;
+              a2[2] += g2 * v2.y;
+              a1[3] += g3 * v3.x;
+              a2[3] += g3 * v3.y;
+              bj = b3 + j + 4;
+            }
+            for (; j < n; j++) {
+              const float4 vj = V[j];
+              const float h = H[bj] + ux * vj.z + uy * vj.w;
+              a1[0] += h * vj.x;
+              a2[0] += h * vj.y;
+              bj += j + 1;
+            }
+            hd[t] = (a1[0] + a1[1]) + (a1[2] + a1[3]);
+            d[t] = (a2[0] + a2[1]) + (a2[2] + a2[3]);
+          }
+          __syncthreads();
+          for (int t = tid; t < n; t += NT) {
+            const float2 ab = AB[t];
+            const float ux = pend ? ab.x : 0.f, uy = pend ? ab.y : 0.f;
+            float a1[4] = {hd[t], 0.f, 0.f, 0.f}, a2[4] = {d[t], 0.f, 0.f, 0.f};
+            float* col = H + packed_floats(t);
+            int i = 0;
+            for (; i + 3 <= t; i += 4) {
+              const float4 v0 = V[i], v1 = V[i + 1], v2 = V[i + 2], v3 = V[i + 3];
+              const float g0 = col[i] + ux * v0.z + uy * v0.w, g1 = col[i + 1] + ux * v1.z + uy * v1.w;
+              const float g2 = col[i + 2] + ux * v2.z + uy * v2.w, g3 = col[i + 3] + ux * v3.z + uy * v3.w;
+              if (pend) {
+                col[i] = g0;
+                col[i + 1] = g1;
+                col[i + 2] = g2;
+                col[i + 3] = g3;
+              }
+              a1[0] += g0 * v0.x;
+              a2[0] += g0 * v0.y;
+              a1[1] += g1 * v1.x;
+              a2[1] += g1 * v1.y;
+              a1[2] += g2 * v2.x;
+              a2[2] += g2 * v2.y;
+              a1[3] += g3 * v3.x;
+              a2[3] += g3 * v3.y;
+            }
+            for (; i <= t; i++) {
+              const float4 vi = V[i];
+              const float h = col[i] + ux * vi.z + uy * vi.w;
+              if (pend) col[i] = h;
+              a1[0] += h * vi.x;
+              a2[0] += h * vi.y;
+            }
+            hd[t] = (a1[0] + a1[1]) + (a1[2] + a1[3]);
+            d[t] = (a2[0] + a2[1]) + (a2[2] + a2[3]);
+          }
+        } else {
+          // full storage, lane t owns column t (= row t by symmetry); coalesced rows,
+          // unrolled by 4 so four global reads are in flight per lane
+          for (int t = tid; t < n; t += NT) {
+            const float2 ab = AB[t];
+            const float ux = pend ? ab.x : 0.f, uy = pend ? ab.y : 0.f;
+            float a1[4] = {0.f, 0.f, 0.f, 0.f}, a2[4] = {0.f, 0.f, 0.f, 0.f};
+            float* col = H + t;
+            int j = 0;
+            for (; j + 3 < n; j += 4) {
+              float* p0 = col + (size_t)j * n;
+              const float h0 = p0[0], h1 = p0[n], h2 = p0[2 * n], h3 = p0[3 * n];
+              const float4 v0 = V[j], v1 = V[j + 1], v2 = V[j + 2], v3 = V[j + 3];
+              const float g0 = h0 + ux * v0.z + uy * v0.w, g1 = h1 + ux * v1.z + uy * v1.w;
+              const float g2 = h2 + ux * v2.z + uy * v2.w, g3 = h3 + ux * v3.z + uy * v3.w;
+              if (pend) {
+                p0[0] = g0;
+                p0[n] = g1;
+                p0[2 * n] = g2;
+                p0[3 * n] = g3;
+              }
+              a1[0] += g0 * v0.x;
+              a2[0] += g0 * v0.y;
+              a1[1] += g1 * v1.x;
+              a2[1] += g1 * v1.y;
+              a1[2] += g2 * v2.x;
+              a2[2] += g2 * v2.y;
+              a1[3] += g3 * v3.x;
+              a2[3] += g3 * v3.y;
+            }
+            for (; j < n; j++) {
+              const float4 vj = V[j];
+              const float h = col[(size_t)j * n] + ux * vj.z + uy * vj.w;
+              if (pend) col[(size_t)j * n] = h;
+              a1[0] += h * vj.x;
+              a2[0] += h * vj.y;
+            }
+            hd[t] = (a1[0] + a1[1]) + (a1[2] + a1[3]);
+            d[t] = (a2[0] + a2[1]) + (a2[2] + a2[3]);
+          }
+        }
+        double v6[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int i = tid; i < n; i += NT) {
+          const float4 v = V[i];
+          const double xi = xo[i], h = hd[i];
+          v6[0] += (double)v.x * xi;
+          v6[1] += (double)v.x * h;
+          v6[2] += (double)v.x * v.x;
+          v6[3] += xi * xi;
+          v6[4] += xi * v.y;
+          v6[5] += h * v.y;
+        }
+        R.template run<6, 0>(v6);
+        const double fac = v6[0], fae = v6[1];
+        const bool upd = fac * fac > EPS_GUARD * v6[2] * v6[3] && fac > 0.0;  // :347
+        if (upd) {
+          // U = A s s^T + Bc (s h^T + h s^T) + Cc h h^T, the expansion of the a a^T term
+          const double fi = 1.0 / fac, fei = 1.0 / fae;
+          const double A = fi + fae * fi * fi, Bc = -fae * fi * fei, Cc = fae * fei * fei - fei;
+          const double xig = v6[4], hdg = v6[5];
+          for (int t = tid; t < n; t += NT) {
+            const double xi = xo[t], h = hd[t];
+            const double al = A * xi + Bc * h, be = Bc * xi + Cc * h;
+            d[t] = -(float)((double)d[t] + al * xig + be * hdg);
+            float4 v = V[t];
+            v.z = (float)xi;
+            v.w = (float)h;
+            V[t] = v;
+            AB[t] = make_float2((float)al, (float)be);
+          }
+          pend = true;
+        } else {
+          for (int t = tid; t < n; t += NT) d[t] = -d[t];
+          pend = false;
+        }
+        __syncthreads();
+      } else {
+        // L-BFGS history and two-loop recursion (mmff_lbfgs_source_tg.metal:249-292)
+        double ys = 0.0;
+        for (int i = tid; i < n; i += NT) ys += (double)V[i].x * xo[i];
+        ys = R.sum(ys);
+        if (ys > 1e-10) {
+          const int slot = hist_idx % M;
+          for (int i = tid; i < n; i += NT) {
+            S[(size_t)slot * mn + i] = xo[i];
+            Y[(size_t)slot * mn + i] = V[i].x;
+          }
+          if (tid == 0) rho[slot] = 1.0 / ys;
+          hist_idx++;
+          if (hist_count < M) hist_count++;
+        }
+        for (int i = tid; i < n; i += NT) hd[i] = V[i].y;
+        __syncthreads();
+        for (int j = hist_count - 1; j >= 0; j--) {
+          int slot = (hist_idx - 1 - (hist_count - 1 - j)) % M;
+          if (slot < 0) slot += M;
+          double sq = 0.0;
+          for (int i = tid; i < n; i += NT) sq += (double)S[(size_t)slot * mn + i] * hd[i];
+          sq = R.sum(sq);
+          const double aj = rho[slot] * sq;
+          if (tid == 0) alf[j] = aj;
+          const float af = (float)aj;
+          for (int i = tid; i < n; i += NT) hd[i] -= af * Y[(size_t)slot * mn + i];
+        }
+        if (hist_count > 0) {
+          int nw = (hist_idx - 1) % M;
+          if (nw < 0) nw += M;
+          double v2b[2] = {0.0, 0.0};
+          for (int i = tid; i < n; i += NT) {
+            const double s_ = S[(size_t)nw * mn + i], y_ = Y[(size_t)nw * mn + i];
+            v2b[0] += s_ * y_;
+            v2b[1] += y_ * y_;
+          }
+          R.template run<2, 0>(v2b);
+          const float gam = (float)(v2b[0] / fmax(v2b[1], 1e-30));
+          for (int i = tid; i < n; i += NT) hd[i] *= gam;
+        }
+        __syncthreads();  // alf / rho visible
+        for (int j = 0; j < hist_count; j++) {
+          int slot = (hist_idx - 1 - (hist_count - 1 - j)) % M;
+          if (slot < 0) slot += M;
+          double yq = 0.0;
+          for (int i = tid; i < n; i += NT) yq += (double)Y[(size_t)slot * mn + i] * hd[i];
+          yq = R.sum(yq);
+          const float cf = (float)(alf[j] - rho[slot] * yq);
+          for (int i = tid; i < n; i += NT) hd[i] += cf * S[(size_t)slot * mn + i];
+        }
+        for (int i = tid; i < n; i += NT) d[i] = -hd[i];
+        __syncthreads();
+      }
+    }
+    for (int i = tid; i < n; i += NT) P.x_out[off + i] = xs[i];
+    if (tid == 0) {
+      P.e_out[c] = E;
+      P.status[c] = status;
+      if (P.counters) {
+        P.counters[3 * c] = iters;
+        P.counters[3 * c + 1] = n_grads;
+        P.counters[3 * c + 2] = n_evals;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------------------------
+// BB fused steepest descent, MEGA family: mmff_metal_kernel.py:845-1075.
+template <int NT>
+__global__ void __launch_bounds__(NT) k_bb_fused(KBatch B, const float* __restrict__ x_in, float* x_out,
+                                                 double* e_out, int max_it, float max_step, float grad_lr) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int NW = NT / 64;
+  const int c = blockIdx.x;
+  if (c >= B.n_confs) return;
+  const int tid = threadIdx.x;
+  const int mx = 3 * B.max_atoms;
+  double* red = (double*)smem;
+  unsigned long long* gacc = (unsigned long long*)(red + 16 * NW);
+  float* xs = (float*)(gacc + mx);
+  float* pp = xs + mx;
+  float* pg = pp + mx;
+  float* g = pg + mx;
+  const MolDesc m = load_desc(B.mol_desc + 16 * B.conf_mol[c]);
+  const int n = 3 * m.n_atoms;
+  const int off = B.conf_off[c];
+  for (int i = tid; i < n; i += NT) {
+    xs[i] = x_in[off + i];
+    pp[i] = xs[i];
+    pg[i] = 0.f;
+  }
+  Reducer<NT> R{red, 0};
+  float alpha = grad_lr;
+  double E_prev = 1e30, E = 0.0;
+  for (int it = 0; it < max_it; it++) {
+    for (int i = tid; i < n; i += NT) gacc[i] = 0ull;
+    __syncthreads();
+    E = R.sum(eval_terms<FAM_MEGA, true, false, NT>(global_view(B, m), m, xs, gacc, 1, 0x7fu, nullptr));
+    if (it > 0 && E > E_prev + 1.0) {  // energy guard :1031-1038
+      for (int i = tid; i < n; i += NT) xs[i] = pp[i];
+      alpha = fmaxf(alpha * 0.5f, 1e-7f);
+      E = E_prev;
+      continue;
+    }
+    for (int i = tid; i < n; i += NT) g[i] = from_fx(gacc[i]);
+    if (it > 0) {  // BB step :1041-1052
+      double v2[2] = {0.0, 0.0};
+      for (int i = tid; i < n; i += NT) {
+        const float si = xs[i] - pp[i], yi = g[i] - pg[i];
+        v2[0] += (double)si * si;
+        v2[1] += (double)si * yi;
+      }
+      R.template run<2, 0>(v2);
+      if (v2[1] > 1e-20) alpha = fminf(fmaxf((float)(v2[0] / v2[1]), 1e-6f), 0.3f);
+    }
+    E_prev = E;
+    for (int i = tid; i < n; i += NT) {
+      pp[i] = xs[i];
+      pg[i] = g[i];
+    }
+    __syncthreads();
+    for (int a = tid; a < m.n_atoms; a += NT) {  // per-atom capped step :1063-1071
+      const float gx = g[3 * a], gy = g[3 * a + 1], gz = g[3 * a + 2];
+      const float gn = sqrtf(gx * gx + gy * gy + gz * gz + 1e-30f);
+      const float s = fminf(max_step / gn, alpha);
+      xs[3 * a] -= s * gx;
+      xs[3 * a + 1] -= s * gy;
+      xs[3 * a + 2] -= s * gz;
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  for (int i = tid; i < n; i += NT) x_out[off + i] = xs[i];
+  if (tid == 0) e_out[c] = E;
+}
+
+// ------------------------------------------------------------------------------------
+// multi_mol lock-step L-BFGS (MEGA family): mmff_optimize_metal_multi_mol
+// (mmff_metal_optimizer.py:217-535). State per conformer in a global workspace; each
+// launch runs `steps` iterations; launches that end at a sync point apply the best-x guard
+// and convergence test and count unconverged conformers in a flag that the next launch
+// reads to skip the rest (the reference's break), so no host round trip is needed.
+struct MMState {
+  float* x;       // [total]
+  float* g;       // [total] scaled gradient
+  float* bx;      // [total] best x
+  float* S;       // [m][total]
+  float* Y;       // [m][total]
+  double* rho;    // [m][n_confs]
+  double* best_e; // [n_confs]
+  double* e_new;  // [n_confs]
+  int32_t* ring;  // [2*n_confs] len, head
+  int32_t* conv;  // [n_confs]
+  int32_t* pci;   // [n_confs]
+  int32_t* flags; // [n_flags]
+  size_t total;
+};
+
+template <int NT>
+__device__ void mm_eval(const KBatch& B, const MolDesc& m, float* xs, unsigned long long* gacc, Reducer<NT>& R,
+                        double& E, int n) {
+  for (int i = threadIdx.x; i < n; i += NT) gacc[i] = 0ull;
+  __syncthreads();
+  E = R.sum(eval_terms<FAM_MEGA, true, false, NT>(global_view(B, m), m, xs, gacc, 1, 0x7fu, nullptr));
+}
+
+// mode 0: initial (x = x_in), mode 1: polish restart (x = best_x); both compute E+G, scale g,
+// reset history; mode 0 also initialises best_e/best_x/conv/pci.
+template <int NT>
+__global__ void __launch_bounds__(NT) k_mm_init(KBatch B, MMState S, const float* x_in, float gsc, int mode,
+                                                int max_iters) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int NW = NT / 64;
+  const int c = blockIdx.x;
+  if (c >= B.n_confs) return;
+  const int mx = 3 * B.max_atoms;
+  double* red = (double*)smem;
+  unsigned long long* gacc = (unsigned long long*)(red + 16 * NW);
+  float* xs = (float*)(gacc + mx);
+  const MolDesc m = load_desc(B.mol_desc + 16 * B.conf_mol[c]);
+  const int n = 3 * m.n_atoms, off = B.conf_off[c];
+  for (int i = threadIdx.x; i < n; i += NT) xs[i] = mode == 0 ? x_in[off + i] : S.bx[off + i];
+  Reducer<NT> R{red, 0};
+  double E;
+  mm_eval<NT>(B, m, xs, gacc, R, E, n);
+  for (int i = threadIdx.x; i < n; i += NT) {
+    S.x[off + i] = xs[i];
+    S.g[off + i] = from_fx(gacc[i]) * gsc;
+    if (mode == 0) S.bx[off + i] = xs[i];
+  }
+  if (threadIdx.x == 0) {
+    S.ring[2 * c] = 0;
+    S.ring[2 * c + 1] = 0;
+    if (mode == 0) {
+      S.best_e[c] = E;
+      S.conv[c] = 0;
+      S.pci[c] = max_iters;
+    }
+    S.e_new[c] = E;
+  }
+}
+
+template <int NT>
+__global__ void __launch_bounds__(NT) k_mm_chunk(KBatch B, MMState S, int steps, int it_end, int do_sync,
+                                                 int phase, const int32_t* prev_flag, int32_t* my_flag, int M,
+                                                 float max_step, float gsc, float tol) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int NW = NT / 64;
+  const int c = blockIdx.x;
+  if (c >= B.n_confs) return;
+  if (prev_flag && *prev_flag == 0) return;  // every conformer converged at the last sync
+  const int tid = threadIdx.x;
+  const int mx = 3 * B.max_atoms;
+  double* red = (double*)smem;
+  unsigned long long* gacc = (unsigned long long*)(red + 16 * NW);
+  float* xs = (float*)(gacc + mx);
+  float* gs = xs + mx;
+  float* q = gs + mx;
+  float* xn = q + mx;
+  double* alf = (double*)(xn + mx);  // [M]
+  const MolDesc m = load_desc(B.mol_desc + 16 * B.conf_mol[c]);
+  const int n = 3 * m.n_atoms, off = B.conf_off[c];
+  const size_t T = S.total;
+  for (int i = tid; i < n; i += NT) {
+    xs[i] = S.x[off + i];
+    gs[i] = S.g[off + i];
+  }
+  int len = S.ring[2 * c], head = S.ring[2 * c + 1];
+  Reducer<NT> R{red, 0};
+  double E = S.e_new[c];
+  __syncthreads();
+  for (int st = 0; st < steps; st++) {
+    // two-loop direction, mmff_metal_optimizer.py:47-74 (history oldest -> newest)
+    for (int i = tid; i < n; i += NT) q[i] = gs[i];
+    for (int h = len - 1; h >= 0; h--) {
+      const int sl = (head + h) % M;
+      double sq = 0.0;
+      for (int i = tid; i < n; i += NT) sq += (double)S.S[sl * T + off + i] * q[i];
+      sq = R.sum(sq);
+      const double a = S.rho[(size_t)sl * B.n_confs + c] * sq;
+      if (tid == 0) alf[h] = a;
+      const float af = (float)a;
+      for (int i = tid; i < n; i += NT) q[i] -= af * S.Y[sl * T + off + i];
+    }
+    double gam = 1.0;
+    if (len > 0) {
+      const int sl = (head + len - 1) % M;
+      double v2[2] = {0.0, 0.0};
+      for (int i = tid; i < n; i += NT) {
+        const double s = S.S[sl * T + off + i], y = S.Y[sl * T + off + i];
+        v2[0] += s * y;
+        v2[1] += y * y;
+      }
+      R.template run<2, 0>(v2);
+      gam = v2[0] / (v2[1] + 1e-30);
+    }
+    const float gf = (float)gam;
+    for (int i = tid; i < n; i += NT) q[i] *= gf;
+    __syncthreads();
+    for (int h = 0; h < len; h++) {
+      const int sl = (head + h) % M;
+      double yr = 0.0;
+      for (int i = tid; i < n; i += NT) yr += (double)S.Y[sl * T + off + i] * q[i];
+      yr = R.sum(yr);
+      const float cf = (float)(alf[h] - S.rho[(size_t)sl * B.n_confs + c] * yr);
+      for (int i = tid; i < n; i += NT) q[i] += cf * S.S[sl * T + off + i];
+    }
+    __syncthreads();
+    // d = -q with the per-atom cap of max_step (:365-373); x_new = x + d
+    for (int a = tid; a < m.n_atoms; a += NT) {
+      const float dx = -q[3 * a], dy = -q[3 * a + 1], dz = -q[3 * a + 2];
+      const float disp = sqrtf(dx * dx + dy * dy + dz * dz + 1e-30f);
+      const float f = fminf(max_step / disp, 1.0f);
+      xn[3 * a] = xs[3 * a] + dx * f;
+      xn[3 * a + 1] = xs[3 * a + 1] + dy * f;
+      xn[3 * a + 2] = xs[3 * a + 2] + dz * f;
+    }
+    __syncthreads();
+    mm_eval<NT>(B, m, xn, gacc, R, E, n);
+    // push (s, y, rho) into the ring (:385-399); rho = 0 on bad curvature
+    int sl;
+    if (len >= M) {
+      sl = head;
+      head = (head + 1) % M;
+    } else {
+      sl = (head + len) % M;
+      len++;
+    }
+    double sy = 0.0;
+    for (int i = tid; i < n; i += NT) {
+      const float gn = from_fx(gacc[i]) * gsc;
+      const float s = xn[i] - xs[i], y = gn - gs[i];
+      S.S[sl * T + off + i] = s;
+      S.Y[sl * T + off + i] = y;
+      sy += (double)s * y;
+      xs[i] = xn[i];
+      gs[i] = gn;
+    }
+    sy = R.sum(sy);
+    if (tid == 0) S.rho[(size_t)sl * B.n_confs + c] = sy > 1e-10 ? 1.0 / (sy + 1e-30) : 0.0;
+    __syncthreads();
+  }
+  if (do_sync) {
+    // best-x guard (:408-412) and convergence (:354-359 main, :477-481 polish)
+    const bool improved = E < S.best_e[c];
+    double gmax = 0.0;
+    for (int a = tid; a < m.n_atoms; a += NT) {
+      const float gx = gs[3 * a], gy = gs[3 * a + 1], gz = gs[3 * a + 2];
+      gmax = fmax(gmax, (double)sqrtf(gx * gx + gy * gy + gz * gz));
+    }
+    gmax = R.max(gmax);
+    if (improved)
+      for (int i = tid; i < n; i += NT) S.bx[off + i] = xs[i];
+    if (tid == 0) {
+      if (improved) S.best_e[c] = E;
+      const bool under = gmax <= (double)tol;
+      if (phase == 0) {
+        if (!S.conv[c] && under) {
+          S.conv[c] = 1;
+          S.pci[c] = it_end;
+        }
+        if (!S.conv[c]) atomicAdd(my_flag, 1);
+      } else if (!under) {
+        atomicAdd(my_flag, 1);
+      }
+    }
+  }
+  for (int i = tid; i < n; i += NT) {
+    S.x[off + i] = xs[i];
+    S.g[off + i] = gs[i];
+  }
+  if (tid == 0) {
+    S.ring[2 * c] = len;
+    S.ring[2 * c + 1] = head;
+    S.e_new[c] = E;
+  }
+}
+
+template <int NT>
+__global__ void __launch_bounds__(NT) k_mm_final(KBatch B, MMState S, float* x_best, double* e_final,
+                                                 float* grad_norm, int32_t* pci, int32_t* conv, int32_t* iters_used,
+                                                 int n_main_flags, int eval_freq, int max_iters) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int NW = NT / 64;
+  const int c = blockIdx.x;
+  if (c >= B.n_confs) return;
+  const int mx = 3 * B.max_atoms;
+  double* red = (double*)smem;
+  unsigned long long* gacc = (unsigned long long*)(red + 16 * NW);
+  float* xs = (float*)(gacc + mx);
+  const MolDesc m = load_desc(B.mol_desc + 16 * B.conf_mol[c]);
+  const int n = 3 * m.n_atoms, off = B.conf_off[c];
+  for (int i = threadIdx.x; i < n; i += NT) xs[i] = S.bx[off + i];
+  __syncthreads();
+  Reducer<NT> R{red, 0};
+  double E = R.sum(eval_terms<FAM_MEGA, false, false, NT>(global_view(B, m), m, xs, gacc, 1, 0x7fu, nullptr));
+  double gmax = 0.0;
+  for (int a = threadIdx.x; a < m.n_atoms; a += NT) {
+    const float gx = S.g[off + 3 * a], gy = S.g[off + 3 * a + 1], gz = S.g[off + 3 * a + 2];
+    gmax = fmax(gmax, (double)sqrtf(gx * gx + gy * gy + gz * gz));
+  }
+  gmax = R.max(gmax);
+  for (int i = threadIdx.x; i < n; i += NT) x_best[off + i] = xs[i];
+  if (threadIdx.x == 0) {
+    e_final[c] = E;
+    grad_norm[c] = (float)gmax;
+    pci[c] = S.pci[c];
+    conv[c] = S.conv[c];
+    if (c == 0) {
+      int used = max_iters;
+      for (int k = 0; k < n_main_flags; k++)
+        if (S.flags[k] == 0) {
+          used = (k + 1) * eval_freq;
+          break;
+        }
+      *iters_used = used;
+    }
+  }
+}
+
+__global__ void k_fill_i32(int32_t* p, int n, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------------------------------
+// host helpers
+struct DevInfo {
+  int n_cu = 0, lds = 0, ok = 0;
+};
+
+DevInfo dev_info() {
+  static thread_local DevInfo cache[16];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return DevInfo{};
+  if (dev < 0 || dev >= 16) return DevInfo{};
+  DevInfo& di = cache[dev];
+  if (!di.ok) {
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, dev) == hipSuccess) {
+      di.n_cu = p.multiProcessorCount;
+      di.lds = (int)p.sharedMemPerBlock;
+      di.ok = 1;
+    }
+  }
+  return di;
+}
+
+// allow dynamic LDS above the 64 KB default for a kernel (gfx950 has 160 KB per workgroup)
+void allow_lds(const void* kernel, size_t bytes) {
+  if (bytes > 65536) hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+int check_batch(const mmffhip_batch* b) {
+  if (!b) return fail(MMFFHIP_ERR_ARG, "batch is NULL");
+  if (b->n_confs < 0 || b->n_mols < 0) return fail(MMFFHIP_ERR_ARG, "negative batch size");
+  if (b->n_confs > 0 && (!b->mol_desc || !b->conf_mol || !b->conf_off))
+    return fail(MMFFHIP_ERR_ARG, "batch descriptor pointers are NULL");
+  if (b->max_atoms <= 0 && b->n_confs > 0) return fail(MMFFHIP_ERR_ARG, "max_atoms must be positive");
+  if (b->max_atoms > 65535) return fail(MMFFHIP_ERR_SIZE, "molecules above 65535 atoms are not supported");
+  return MMFFHIP_OK;
+}
+
+size_t eg_lds(int nt, int max_atoms) { return 16 * (nt / 64) * 8 + (size_t)3 * max_atoms * (8 + 4) + 16; }
+
+template <int FAM>
+int launch_eg(const KBatch& kb, const float* x, double* e, double* et, float* g, uint32_t mask, int diel,
+              hipStream_t s) {
+  const size_t lds = eg_lds(256, kb.max_atoms);
+  if ((int)lds > dev_info().lds) return fail(MMFFHIP_ERR_SIZE, "molecule too large for the LDS of one workgroup");
+  allow_lds((const void*)&k_energy_grad<FAM, 256>, lds);
+  hipLaunchKernelGGL((k_energy_grad<FAM, 256>), dim3(kb.n_confs), dim3(256), lds, s, kb, x, e, et, g, mask, diel);
+  HIP_TRY(hipGetLastError());
+  return MMFFHIP_OK;
+}
+
+int round4(int v) { return (v + 3) & ~3; }
+
+struct NkPlan {
+  int nt = 256;
+  bool hlds = true;
+  bool tlds = false;
+  int tbytes = 0;
+  size_t lds = 0;
+  int grid = 0;
+  size_t slot_floats = 0;
+  int max_n = 0;
+};
+
+template <int ALGO, bool HLDS, bool TLDS, int NT>
+const void* nk_kernel_ptr() {
+  return (const void*)&k_minimize_nk<ALGO, HLDS, TLDS, NT>;
+}
+
+#define NK_FOR_EACH(X)                                                                                   \
+  X(0, true, true, 64) X(0, true, true, 128) X(0, true, true, 192) X(0, true, true, 256)                 \
+  X(0, true, false, 64) X(0, true, false, 128) X(0, true, false, 192) X(0, true, false, 256)             \
+  X(0, true, false, 320) X(0, true, false, 512)                                                          \
+  X(0, false, false, 256) X(0, false, false, 320) X(0, false, false, 512)                                \
+  X(1, true, true, 64) X(1, true, true, 128) X(1, true, true, 256)                                       \
+  X(1, true, false, 128) X(1, true, false, 256) X(1, true, false, 512)                                   \
+  X(1, false, false, 256) X(1, false, false, 512)
+
+const void* nk_kernel(int algo, bool hlds, bool tlds, int nt) {
+#define NKSEL(A, H, TL, T) \
+  if (algo == A && hlds == H && tlds == TL && nt == T) return nk_kernel_ptr<A, H, TL, T>();
+  NK_FOR_EACH(NKSEL)
+#undef NKSEL
+  return nullptr;
+}
+
+int make_nk_plan(const mmffhip_batch* b, const mmffhip_nk_opts* o, NkPlan& pl) {
+  if (!o) return fail(MMFFHIP_ERR_ARG, "options are NULL");
+  if (o->algo != MMFFHIP_BFGS && o->algo != MMFFHIP_LBFGS) return fail(MMFFHIP_ERR_ARG, "unknown algo");
+  if (o->algo == MMFFHIP_LBFGS && (o->lbfgs_m < 1 || o->lbfgs_m > LBFGS_MAX_M))
+    return fail(MMFFHIP_ERR_ARG, "lbfgs_m must be in [1, 16]");
+  if (o->diel != 1 && o->diel != 2) return fail(MMFFHIP_ERR_ARG, "diel must be 1 or 2");
+  if (o->max_iters < 0) return fail(MMFFHIP_ERR_ARG, "max_iters must be >= 0");
+  DevInfo di = dev_info();
+  if (!di.ok) return fail(MMFFHIP_ERR_DEVICE, "no HIP device");
+  pl.nt = o->block > 0 ? o->block : (3 * b->max_atoms > 256 ? 512 : 256);
+  if (pl.nt != 64 && pl.nt != 128 && pl.nt != 192 && pl.nt != 256 && pl.nt != 320 && pl.nt != 512)
+    return fail(MMFFHIP_ERR_ARG, "block must be 64, 128, 192, 256, 320 or 512");
+  pl.max_n = round4(3 * std::max(b->max_atoms, 1));
+  const int m = o->algo == MMFFHIP_LBFGS ? o->lbfgs_m : 0;
+  const size_t lds_in = nk_lds_bytes(o->algo, true, pl.nt, pl.max_n, m);
+  const size_t lds_out = nk_lds_bytes(o->algo, false, pl.nt, pl.max_n, m);
+  if (o->hess_place == MMFFHIP_HESS_LDS) {
+    pl.hlds = true;
+  } else if (o->hess_place == MMFFHIP_HESS_GLOBAL) {
+    pl.hlds = false;
+  } else {
+    pl.hlds = lds_in <= (size_t)di.lds;
+  }
+  pl.lds = pl.hlds ? lds_in : lds_out;
+  if (pl.lds > (size_t)di.lds) return fail(MMFFHIP_ERR_SIZE, "plan exceeds the LDS of one workgroup");
+  // stage term tables in LDS when asked and when they fit next to the Hessian
+  if (o->table_lds_bytes > 0) {
+    const size_t with = nk_lds_bytes(o->algo, pl.hlds, pl.nt, pl.max_n, m, (size_t)o->table_lds_bytes);
+    if (with <= (size_t)di.lds && nk_kernel(o->algo, pl.hlds, true, pl.nt)) {
+      pl.tlds = true;
+      pl.tbytes = ((o->table_lds_bytes + 15) / 16) * 16;
+      pl.lds = with;
+    }
+  }
+  const void* kp = nk_kernel(o->algo, pl.hlds, pl.tlds, pl.nt);
+  if (!kp) return fail(MMFFHIP_ERR_ARG, "no kernel instantiated for this (algo, placement, block) combination");
+  allow_lds(kp, pl.lds);
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kp, pl.nt, pl.lds) != hipSuccess || per_cu < 1)
+    per_cu = 1;
+  pl.grid = std::max(1, std::min(b->n_confs, di.n_cu * per_cu));
+  pl.slot_floats = 0;
+  if (!pl.hlds) {
+    pl.slot_floats = o->algo == MMFFHIP_LBFGS ? (size_t)2 * m * pl.max_n : (size_t)pl.max_n * pl.max_n;
+    pl.slot_floats = (pl.slot_floats + 63) & ~(size_t)63;
+  }
+  return MMFFHIP_OK;
+}
+
+size_t nk_ws_bytes(const NkPlan& pl) { return 256 + (size_t)pl.grid * pl.slot_floats * 4; }
+
+template <int ALGO, bool HLDS, bool TLDS, int NT>
+void launch_nk_t(const KBatch& kb, const NkParams& P, const NkPlan& pl, hipStream_t s) {
+  hipLaunchKernelGGL((k_minimize_nk<ALGO, HLDS, TLDS, NT>), dim3(pl.grid), dim3(NT), pl.lds, s, kb, P);
+}
+
+void launch_nk(int algo, const KBatch& kb, const NkParams& P, const NkPlan& pl, hipStream_t s) {
+#define NKL(A, H, TL, T)                                          \
+  if (algo == A && pl.hlds == H && pl.tlds == TL && pl.nt == T) { \
+    launch_nk_t<A, H, TL, T>(kb, P, pl, s);                       \
+    return;                                                       \
+  }
+  NK_FOR_EACH(NKL)
+#undef NKL
+}
+
+}  // namespace
+
+// ====================================================================================
+// extern "C" ABI
+extern "C" {
+
+int mmffhip_abi_version(void) { return MMFFHIP_ABI_VERSION; }
+
+const char* mmffhip_last_error(void) { return g_last_error.c_str(); }
+
+int mmffhip_device_info(int32_t* n_cu, int32_t* n_xcd, int32_t* lds_bytes) {
+  DevInfo di = dev_info();
+  if (!di.ok) return fail(MMFFHIP_ERR_DEVICE, "no HIP device");
+  if (n_cu) *n_cu = di.n_cu;
+  if (n_xcd) *n_xcd = 8;
+  if (lds_bytes) *lds_bytes = di.lds;
+  return MMFFHIP_OK;
+}
+
+int mmffhip_energy_grad(const mmffhip_batch* b, int family, int diel, const float* x, double* e_out,
+                        double* e_terms, float* g_out, uint32_t term_mask, void* stream) {
+  int rc = check_batch(b);
+  if (rc) return rc;
+  if (b->n_confs == 0) return MMFFHIP_OK;
+  if (!x || !e_out) return fail(MMFFHIP_ERR_ARG, "x and e_out are required");
+  if (diel != 1 && diel != 2) return fail(MMFFHIP_ERR_ARG, "diel must be 1 or 2");
+  if (family == MMFFHIP_FAMILY_MEGA && diel != 1)
+    return fail(MMFFHIP_ERR_ARG, "the MEGA family has a constant dielectric only");
+  KBatch kb = to_kbatch(b);
+  hipStream_t s = (hipStream_t)stream;
+  if (family == MMFFHIP_FAMILY_TG) return launch_eg<FAM_TG>(kb, x, e_out, e_terms, g_out, term_mask, diel, s);
+  if (family == MMFFHIP_FAMILY_MEGA) return launch_eg<FAM_MEGA>(kb, x, e_out, e_terms, g_out, term_mask, 1, s);
+  return fail(MMFFHIP_ERR_ARG, "unknown family");
+}
+
+int mmffhip_minimize_nk_workspace(const mmffhip_batch* b, const mmffhip_nk_opts* o, size_t* bytes) {
+  int rc = check_batch(b);
+  if (rc) return rc;
+  if (!bytes) return fail(MMFFHIP_ERR_ARG, "bytes is NULL");
+  if (b->n_confs == 0) {
+    *bytes = 256;
+    return MMFFHIP_OK;
+  }
+  NkPlan pl;
+  rc = make_nk_plan(b, o, pl);
+  if (rc) return rc;
+  *bytes = nk_ws_bytes(pl);
+  return MMFFHIP_OK;
+}
+
+int mmffhip_minimize_nk(const mmffhip_batch* b, const mmffhip_nk_opts* o, const float* x_in, float* x_out,
+                        double* e_out, int32_t* status_out, int32_t* counters, const int32_t* order,
+                        void* workspace, size_t workspace_bytes, void* stream) {
+  int rc = check_batch(b);
+  if (rc) return rc;
+  if (b->n_confs == 0) return MMFFHIP_OK;
+  if (!x_in || !x_out || !e_out || !status_out || !workspace)
+    return fail(MMFFHIP_ERR_ARG, "x_in, x_out, e_out, status_out and workspace are required");
+  NkPlan pl;
+  rc = make_nk_plan(b, o, pl);
+  if (rc) return rc;
+  if (workspace_bytes < nk_ws_bytes(pl)) return fail(MMFFHIP_ERR_SIZE, "workspace too small");
+  hipStream_t s = (hipStream_t)stream;
+  int32_t* queue = (int32_t*)workspace;
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, queue, 1, 0);
+  HIP_TRY(hipGetLastError());
+  NkParams P;
+  P.x_in = x_in;
+  P.x_out = x_out;
+  P.e_out = e_out;
+  P.status = status_out;
+  P.counters = counters;
+  P.order = order;
+  P.queue = queue;
+  P.hglob = (float*)((char*)workspace + 256);
+  P.slot_floats = pl.slot_floats;
+  P.precond = o->precond;
+  P.max_iters = o->max_iters;
+  P.grad_tol = o->grad_tol;
+  P.diel = o->diel;
+  P.m = o->algo == MMFFHIP_LBFGS ? o->lbfgs_m : 1;
+  P.max_n = pl.max_n;
+  P.ls_grad = o->ls_grad;
+  P.tbytes = pl.tbytes;
+  KBatch kb = to_kbatch(b);
+  launch_nk(o->algo, kb, P, pl, s);
+  HIP_TRY(hipGetLastError());
+  return MMFFHIP_OK;
+}
+
+int mmffhip_bb_fused(const mmffhip_batch* b, const float* x_in, float* x_out, double* e_out, int32_t max_iters,
+                     float max_step, float grad_lr, void* stream) {
+  int rc = check_batch(b);
+  if (rc) return rc;
+  if (b->n_confs == 0) return MMFFHIP_OK;
+  if (!x_in || !x_out || !e_out) return fail(MMFFHIP_ERR_ARG, "x_in, x_out and e_out are required");
+  if (max_iters < 0) return fail(MMFFHIP_ERR_ARG, "max_iters must be >= 0");
+  const size_t lds = 16 * 4 * 8 + (size_t)3 * b->max_atoms * (8 + 4 * 4) + 16;
+  if ((int)lds > dev_info().lds) return fail(MMFFHIP_ERR_SIZE, "molecule too large for the LDS of one workgroup");
+  KBatch kb = to_kbatch(b);
+  allow_lds((const void*)&k_bb_fused<256>, lds);
+  hipLaunchKernelGGL((k_bb_fused<256>), dim3(b->n_confs), dim3(256), lds, (hipStream_t)stream, kb, x_in, x_out,
+                     e_out, (int)max_iters, max_step, grad_lr);
+  HIP_TRY(hipGetLastError());
+  return MMFFHIP_OK;
+}
+
+int mmffhip_multimol_workspace(const mmffhip_batch* b, int32_t lbfgs_m, size_t* bytes) {
+  int rc = check_batch(b);
+  if (rc) return rc;
+  if (!bytes) return fail(MMFFHIP_ERR_ARG, "bytes is NULL");
+  if (lbfgs_m < 1 || lbfgs_m > 64) return fail(MMFFHIP_ERR_ARG, "lbfgs_m must be in [1, 64]");
+  // the total coordinate count is not known on the host without a device read; the
+  // workspace is sized with n_confs * 3 * max_atoms as an upper bound
+  const size_t T = (size_t)b->n_confs * 3 * b->max_atoms;
+  const size_t C = (size_t)b->n_confs;
+  auto r256 = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  size_t by = 0;
+  by += 3 * r256(T * 4) + 2 * r256(T * 4 * (size_t)lbfgs_m);
+  by += r256(C * 8 * (size_t)lbfgs_m) + 2 * r256(C * 8) + r256(C * 8) + 2 * r256(C * 4);
+  by += r256(4096 * 4);
+  *bytes = by + 1024;
+  return MMFFHIP_OK;
+}
+
+int mmffhip_multimol_run(const mmffhip_batch* b, const float* x_in, float* x_best, double* e_final, float* grad_norm,
+                         int32_t* per_conf_iters, int32_t* converged, int32_t* iters_used, int32_t max_iters,
+                         int32_t eval_freq, float max_step_per_atom, int32_t lbfgs_m, float grad_scale,
+                         float grad_tol, void* workspace, size_t workspace_bytes, void* stream) {
+  int rc = check_batch(b);
+  if (rc) return rc;
+  if (b->n_confs == 0) return MMFFHIP_OK;
+  if (!x_in || !x_best || !e_final || !grad_norm || !per_conf_iters || !converged || !iters_used || !workspace)
+    return fail(MMFFHIP_ERR_ARG, "NULL output or workspace");
+  if (max_iters < 0 || eval_freq < 1) return fail(MMFFHIP_ERR_ARG, "max_iters >= 0 and eval_freq >= 1 required");
+  size_t need = 0;
+  rc = mmffhip_multimol_workspace(b, lbfgs_m, &need);
+  if (rc) return rc;
+  if (workspace_bytes < need) return fail(MMFFHIP_ERR_SIZE, "workspace too small");
+  const int n_main = (max_iters + eval_freq - 1) / eval_freq;
+  const int polish = max_iters / 4;
+  const int n_pol = (polish + eval_freq - 1) / eval_freq;
+  if (n_main + n_pol > 4096) return fail(MMFFHIP_ERR_ARG, "too many sync points (max_iters / eval_freq > 4096)");
+  const size_t T = (size_t)b->n_confs * 3 * b->max_atoms;
+  const size_t C = (size_t)b->n_confs;
+  char* p = (char*)workspace;
+  auto take = [&](size_t bytes) {
+    char* q = p;
+    p += (bytes + 255) & ~(size_t)255;
+    return (void*)q;
+  };
+  MMState S;
+  S.total = T;
+  S.x = (float*)take(T * 4);
+  S.g = (float*)take(T * 4);
+  S.bx = (float*)take(T * 4);
+  S.S = (float*)take(T * 4 * lbfgs_m);
+  S.Y = (float*)take(T * 4 * lbfgs_m);
+  S.rho = (double*)take(C * 8 * lbfgs_m);
+  S.best_e = (double*)take(C * 8);
+  S.e_new = (double*)take(C * 8);
+  S.ring = (int32_t*)take(C * 8);
+  S.conv = (int32_t*)take(C * 4);
+  S.pci = (int32_t*)take(C * 4);
+  S.flags = (int32_t*)take(4096 * 4);
+  if ((size_t)(p - (char*)workspace) > workspace_bytes) return fail(MMFFHIP_ERR_SIZE, "workspace too small");
+  hipStream_t s = (hipStream_t)stream;
+  KBatch kb = to_kbatch(b);
+  constexpr int NT = 256;
+  const int mx = 3 * b->max_atoms;
+  const size_t lds_init = 16 * (NT / 64) * 8 + (size_t)mx * 12 + 16;
+  const size_t lds_chunk = 16 * (NT / 64) * 8 + (size_t)mx * (8 + 16) + 8 * (size_t)lbfgs_m + 32;
+  if ((int)lds_chunk > dev_info().lds) return fail(MMFFHIP_ERR_SIZE, "molecule too large for one workgroup");
+  const int nfl = n_main + n_pol;
+  allow_lds((const void*)&k_mm_init<NT>, lds_init);
+  allow_lds((const void*)&k_mm_chunk<NT>, lds_chunk);
+  allow_lds((const void*)&k_mm_final<NT>, lds_init);
+  hipLaunchKernelGGL(k_fill_i32, dim3((4096 + 255) / 256), dim3(256), 0, s, S.flags, 4096, 0);
+  hipLaunchKernelGGL((k_mm_init<NT>), dim3(b->n_confs), dim3(NT), lds_init, s, kb, S, x_in, grad_scale, 0,
+                     (int)max_iters);
+  HIP_TRY(hipGetLastError());
+  int it = 0, k = 0;
+  while (it < max_iters) {
+    int steps = std::min(eval_freq, max_iters - it);
+    it += steps;
+    const bool sync = (it % eval_freq) == 0;
+    hipLaunchKernelGGL((k_mm_chunk<NT>), dim3(b->n_confs), dim3(NT), lds_chunk, s, kb, S, steps, it, sync ? 1 : 0,
+                       0, k > 0 ? S.flags + (k - 1) : (const int32_t*)nullptr, S.flags + k, (int)lbfgs_m,
+                       max_step_per_atom, grad_scale, grad_tol);
+    if (sync) k++;
+  }
+  HIP_TRY(hipGetLastError());
+  hipLaunchKernelGGL((k_mm_init<NT>), dim3(b->n_confs), dim3(NT), lds_init, s, kb, S, x_in, grad_scale, 1,
+                     (int)max_iters);
+  it = 0;
+  int kp = n_main;
+  while (it < polish) {
+    int steps = std::min(eval_freq, polish - it);
+    it += steps;
+    const bool sync = (it % eval_freq) == 0;
+    hipLaunchKernelGGL((k_mm_chunk<NT>), dim3(b->n_confs), dim3(NT), lds_chunk, s, kb, S, steps, it, sync ? 1 : 0,
+                       1, kp > n_main ? S.flags + (kp - 1) : (const int32_t*)nullptr, S.flags + kp, (int)lbfgs_m,
+                       max_step_per_atom, grad_scale, grad_tol);
+    if (sync) kp++;
+  }
+  HIP_TRY(hipGetLastError());
+  (void)nfl;
+  hipLaunchKernelGGL((k_mm_final<NT>), dim3(b->n_confs), dim3(NT), lds_init, s, kb, S, x_best, e_final, grad_norm,
+                     per_conf_iters, converged, iters_used, (int)(max_iters / eval_freq), (int)eval_freq,
+                     (int)max_iters);
+  HIP_TRY(hipGetLastError());
+  return MMFFHIP_OK;
+}
+
+}  // extern "C"

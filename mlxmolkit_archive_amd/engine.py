@@ -1,0 +1,184 @@
+"""Device-side entry points: thin wrappers over the C ABI operating on torch tensors.
+
+Everything here keeps data on the GPU. The drop-in APIs (minimize.py, optimizer.py,
+kernel.py, pipeline.py) are built on these functions.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from .packing import DeviceBatch, HostBatch, lpt_order, pack
+
+TERM_NAMES = ("bond", "angle", "strbend", "oop", "torsion", "vdw", "ele")
+
+
+def _p(t) -> int:
+    return 0 if t is None else int(t.data_ptr())
+
+
+def device_batch(params_list, conf_counts, device="cuda") -> DeviceBatch:
+    torch = _lib.require_gpu()
+    return DeviceBatch(pack(params_list, conf_counts), device=device)
+
+
+def energy_grad(db: DeviceBatch, x, family: int = _lib.FAMILY_TG, diel: int = 1, mask: int = 0x7F,
+                grad: bool = True, terms: bool = False, stream=None):
+    """(E float64 [C], G float32 like x or None, E_terms float64 [C,7] or None) on the GPU."""
+    torch = _lib.require_gpu()
+    L = _lib.load()
+    assert x.dtype == torch.float32 and x.is_cuda and x.numel() == db.host.total_coords
+    C_ = db.n_confs
+    e = torch.empty(C_, dtype=torch.float64, device=x.device)
+    g = torch.empty_like(x) if grad else None
+    et = torch.empty((C_, 7), dtype=torch.float64, device=x.device) if terms else None
+    _lib.check(L.mmffhip_energy_grad(C.byref(db.desc), family, diel, _p(x), _p(e), _p(et), _p(g), mask,
+                                     _lib.stream_ptr(stream)))
+    return e, g, et
+
+
+@dataclass
+class NkResult:
+    x: "object"          # float32 [total] on device
+    energy: "object"     # float64 [C]
+    status: "object"     # int32 [C]
+    counters: "object"   # int32 [C,3] iterations, gradient evals, energy evals
+
+
+def nk_options(max_iters=500, grad_tol=1e-4, use_lbfgs=False, lbfgs_m=8, diel=1, hess_place=_lib.HESS_AUTO,
+               block=0, ls_grad=False, precond=None, table_lds_bytes=0) -> _lib.NkOpts:
+    o = _lib.NkOpts()
+    o.algo = _lib.LBFGS if use_lbfgs else _lib.BFGS
+    o.max_iters = int(max_iters)
+    o.grad_tol = float(grad_tol)
+    o.diel = int(diel)
+    o.lbfgs_m = int(lbfgs_m)
+    o.hess_place = int(hess_place)
+    o.block = int(block)
+    o.ls_grad = int(bool(ls_grad))
+    o.precond = _p(precond)
+    o.table_lds_bytes = int(table_lds_bytes)
+    return o
+
+
+def table_lds_bytes(hb: HostBatch) -> np.ndarray:
+    """Per-molecule bytes of the LDS staging layout of its term tables (csrc table_lds_bytes)."""
+    d = hb.mol_desc.reshape(-1, 16).astype(np.int64)
+    nb, na, nsb, noop, nt, npair = d[:, 2], d[:, 4], d[:, 6], d[:, 8], d[:, 10], d[:, 12]
+    return 16 * (nt + nsb + npair) + 8 * (2 * na + nsb + noop + nt + nb) + 4 * (nb + nsb + noop + npair)
+
+
+# Size buckets for the nk minimizer: (max 3N, threads per workgroup, Hessian placement).
+# Each bucket is its own persistent launch whose LDS footprint (and hence occupancy) is
+# sized for its largest molecule. 270 = largest n whose packed FP32 inverse Hessian plus
+# vectors fits the 160 KB of LDS a gfx950 workgroup can own.
+DEFAULT_BUCKETS = ((96, 128, _lib.HESS_AUTO), (192, 192, _lib.HESS_AUTO), (270, 320, _lib.HESS_LDS),
+                   (1 << 20, 512, _lib.HESS_GLOBAL))
+LBFGS_BUCKETS = ((192, 128, _lib.HESS_LDS), (600, 256, _lib.HESS_LDS), (1 << 20, 512, _lib.HESS_AUTO))
+
+
+class NkRunner:
+    """Pre-planned minimize_nk launches for one batch (one launch per size bucket).
+
+    ``buckets``: sequence of (max 3N, block, hess_place); conformers go to the first bucket
+    whose bound covers them. ``order``: optional global conformer order (default: longest
+    first inside each bucket).
+    """
+
+    def __init__(self, db: DeviceBatch, order: Optional[np.ndarray] = None, buckets=None, **opts):
+        torch = _lib.require_gpu()
+        self.db = db
+        self._precond = opts.get("precond")
+        use_lbfgs = bool(opts.get("use_lbfgs", False))
+        user_block = int(opts.pop("block", 0) or 0)
+        user_place = int(opts.pop("hess_place", _lib.HESS_AUTO) or 0)
+        stage = bool(opts.pop("stage_tables", True))
+        tbytes_mol = table_lds_bytes(db.host)
+        if buckets is None:
+            buckets = LBFGS_BUCKETS if use_lbfgs else DEFAULT_BUCKETS
+        hb = db.host
+        n3 = 3 * hb.n_atoms[hb.conf_mol]
+        if order is None:
+            order = lpt_order(hb)
+        order = np.asarray(order, np.int32)
+        dev = db.t["conf_mol"].device
+        L = _lib.load()
+        self.launches = []
+        assigned = np.zeros(hb.n_confs, bool)
+        for bound, block, place in buckets:
+            sel = order[(n3[order] <= bound) & ~assigned[order]]
+            if len(sel) == 0:
+                continue
+            assigned[sel] = True
+            desc = _lib.Batch.from_buffer_copy(db.desc)
+            desc.n_confs = len(sel)
+            desc.max_atoms = int(hb.n_atoms[hb.conf_mol[sel]].max())
+            tb = int(tbytes_mol[hb.conf_mol[sel]].max()) if stage else 0
+            o = nk_options(block=user_block or block, hess_place=user_place or place, table_lds_bytes=tb, **opts)
+            nbytes = C.c_size_t()
+            _lib.check(L.mmffhip_minimize_nk_workspace(C.byref(desc), C.byref(o), C.byref(nbytes)))
+            ws = torch.empty(int(nbytes.value), dtype=torch.uint8, device=dev)
+            ordt = torch.from_numpy(np.ascontiguousarray(sel)).to(dev)
+            self.launches.append(dict(desc=desc, opts=o, ws=ws, ws_bytes=int(nbytes.value), order=ordt,
+                                      n=len(sel), max_n=3 * desc.max_atoms, tbytes=tb))
+        assert assigned.all()
+        C_ = db.n_confs
+        self.e = torch.empty(C_, dtype=torch.float64, device=dev)
+        self.st = torch.empty(C_, dtype=torch.int32, device=dev)
+        self.cnt = torch.empty((C_, 3), dtype=torch.int32, device=dev)
+
+    def run(self, x_in, x_out=None, stream=None) -> NkResult:
+        torch = _lib.require_gpu()
+        assert x_in.dtype == torch.float32 and x_in.numel() == self.db.host.total_coords
+        if x_out is None:
+            x_out = torch.empty_like(x_in)
+        L = _lib.load()
+        sp = _lib.stream_ptr(stream)
+        for ln in self.launches:
+            _lib.check(L.mmffhip_minimize_nk(C.byref(ln["desc"]), C.byref(ln["opts"]), _p(x_in), _p(x_out),
+                                             _p(self.e), _p(self.st), _p(self.cnt), _p(ln["order"]), _p(ln["ws"]),
+                                             ln["ws_bytes"], sp))
+        return NkResult(x_out, self.e, self.st, self.cnt)
+
+
+def minimize_nk(db: DeviceBatch, x, **opts) -> NkResult:
+    r = NkRunner(db, **opts)
+    res = r.run(x)
+    return NkResult(res.x, res.energy.clone(), res.status.clone(), res.counters.clone())
+
+
+def bb_fused(db: DeviceBatch, x, max_iters=500, max_step=0.3, grad_lr=0.1, stream=None):
+    torch = _lib.require_gpu()
+    L = _lib.load()
+    x_out = torch.empty_like(x)
+    e = torch.empty(db.n_confs, dtype=torch.float64, device=x.device)
+    _lib.check(L.mmffhip_bb_fused(C.byref(db.desc), _p(x), _p(x_out), _p(e), int(max_iters), float(max_step),
+                                  float(grad_lr), _lib.stream_ptr(stream)))
+    return x_out, e
+
+
+def multimol(db: DeviceBatch, x, max_iters=200, eval_freq=10, max_step_per_atom=0.3, lbfgs_m=10, grad_scale=0.1,
+             grad_tol=1e-4, stream=None):
+    """Lock-step L-BFGS of mmff_optimize_metal_multi_mol, fully on the device."""
+    torch = _lib.require_gpu()
+    L = _lib.load()
+    nbytes = C.c_size_t()
+    _lib.check(L.mmffhip_multimol_workspace(C.byref(db.desc), int(lbfgs_m), C.byref(nbytes)))
+    dev = x.device
+    ws = torch.empty(int(nbytes.value), dtype=torch.uint8, device=dev)
+    C_ = db.n_confs
+    xb = torch.empty_like(x)
+    e = torch.empty(C_, dtype=torch.float64, device=dev)
+    gn = torch.empty(C_, dtype=torch.float32, device=dev)
+    pci = torch.empty(C_, dtype=torch.int32, device=dev)
+    conv = torch.empty(C_, dtype=torch.int32, device=dev)
+    used = torch.zeros(1, dtype=torch.int32, device=dev)
+    _lib.check(L.mmffhip_multimol_run(C.byref(db.desc), _p(x), _p(xb), _p(e), _p(gn), _p(pci), _p(conv), _p(used),
+                                      int(max_iters), int(eval_freq), float(max_step_per_atom), int(lbfgs_m),
+                                      float(grad_scale), float(grad_tol), _p(ws), int(nbytes.value),
+                                      _lib.stream_ptr(stream)))
+    return dict(x=xb, energy=e, grad_norm=gn, per_conf_iters=pci, converged=conv, iters_used=used)

@@ -1,0 +1,70 @@
+"""Seeded benchmark / test workloads for the minimizer.
+
+Each workload is a deterministic function of its name and seed (base 20261202):
+
+=========  ===================  ===========================================================
+name       molecules x confs    molecules
+=========  ===================  ===========================================================
+small      50 x 50              10-25 heavy atoms, <= 1 ring (about 20-50 atoms with H)
+medium     1,000 x 10           15-40 heavy atoms, <= 1 ring (about 30-90 atoms)
+ragged     10,000 x 20          heavy atoms log-uniform 8-60, <= 3 fused/spiro rings
+chain      2,000 x 8            flexible chains of 60-150 heavy atoms, 0-2 macrocycles
+=========  ===================  ===========================================================
+
+Start geometries: the package's embedder (seed per conformer) plus N(0, 0.1 A) noise per
+coordinate. Molecule and conformer counts can be scaled down for tests.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+from typing import List, Optional
+
+import numpy as np
+
+from . import molgen
+from .embed import embed
+from .mmff_params import MMFFParams, extract_mmff_params
+
+SPECS = {
+    "small": ("small", 50, 50),
+    "medium": ("medium", 1000, 10),
+    "ragged": ("ragged", 10000, 20),
+    "chain": ("chain", 2000, 8),
+}
+
+
+@dataclass
+class Workload:
+    name: str
+    mols: list
+    params: List[MMFFParams]
+    conf_counts: List[int]
+    x: np.ndarray  # float32 flat start coordinates (sum over conformers of 3N)
+
+    @property
+    def n_confs(self) -> int:
+        return int(sum(self.conf_counts))
+
+    def atoms_per_conf(self) -> np.ndarray:
+        return np.repeat([p.n_atoms for p in self.params], self.conf_counts)
+
+
+def build(name: str, n_mols: Optional[int] = None, n_confs: Optional[int] = None,
+          seed: int = molgen.SEED_BASE, noise: float = 0.1, variant: str = "MMFF94",
+          n_threads: int = 0) -> Workload:
+    fam, M, K = SPECS[name]
+    M = M if n_mols is None else n_mols
+    K = K if n_confs is None else n_confs
+    mols = molgen.generate(fam, M, seed=seed)
+    params = []
+    for m in mols:
+        # the non-bonded 100 A filter of extract_mmff_params needs a conformer; the
+        # generated molecules are far smaller than 100 A, so a zero geometry keeps every pair
+        m.set_conformers(np.zeros((1, m.GetNumAtoms(), 3)))
+        params.append(extract_mmff_params(m, mmff_variant=variant))
+    counts = [K] * M
+    ids = np.arange(M * K, dtype=np.uint64)
+    seeds = (np.uint64(seed) * np.uint64(1000003) + np.uint64(molgen._FAMILY_ID[fam]) * np.uint64(7919) * np.uint64(
+        10 ** 9) + ids)
+    x = embed(params, [m.atomic_numbers() for m in mols], counts, seeds=seeds, noise=noise, n_threads=n_threads)
+    return Workload(name=name, mols=mols, params=params, conf_counts=counts, x=x)
