@@ -176,6 +176,14 @@ class MMFFTyper:
                 if self.ring_size[a] == 0 or len(r) < self.ring_size[a]:
                     self.ring_size[a] = len(r)
         self.arom = self._perceive_aromatic()
+        # bonds inside a perceived aromatic ring count as order 1.5 whatever the input
+        # (Kekule) order was
+        self._arom_bond = set()
+        for r in self.rings:
+            if len(r) in (5, 6) and all(self.arom[a] for a in r):
+                for k in range(len(r)):
+                    a, b = r[k], r[(k + 1) % len(r)]
+                    self._arom_bond.add((min(a, b), max(a, b)))
         self.types = self._assign_types()
         self.charges = self._charges()
 
@@ -313,6 +321,11 @@ class MMFFTyper:
         return q
 
     # -- helpers -----------------------------------------------------------
+    def bond_order(self, i: int, j: int) -> float:
+        if (min(i, j), max(i, j)) in self._arom_bond:
+            return 1.5
+        return self.mol.bond_order(i, j)
+
     def is_linear(self, i: int) -> bool:
         return HYB.get(self.types[i], 3) == 1
 
@@ -323,7 +336,7 @@ class MMFFTyper:
 
     # -- parameters --------------------------------------------------------
     def bond(self, i: int, j: int) -> Tuple[float, float]:
-        o = self.mol.bond_order(i, j)
+        o = self.bond_order(i, j)
         ci, cj = _cls(self.types[i], self.z[i]), _cls(self.types[j], self.z[j])
         key_o = 1.5 if o == 1.5 else int(o)
         for key in ((ci, cj, key_o), (cj, ci, key_o)):
@@ -420,7 +433,7 @@ class MMFFTyper:
         if self.is_linear(j) or self.is_linear(k):
             return None
         hj, hk = self.hyb(j), self.hyb(k)
-        o = self.mol.bond_order(j, k)
+        o = self.bond_order(j, k)
         tj, tk = self.types[j], self.types[k]
         heavy_ends = int(self.z[i] != 1) + int(self.z[l] != 1)
         soft = self.variant == "MMFF94s"

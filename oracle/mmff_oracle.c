@@ -137,7 +137,9 @@ static double tg_strbend(const double *x, double *g, int i, int j, int k, double
     double d1 = 1.0 / dmax(a.inv_d1, 1e-8), d2 = 1.0 / dmax(a.inv_d2, 1e-8);
     double dt = a.theta_deg - t0;
     double drij = d1 - r0ij, drkj = d2 - r0kj;
-    double e = MDYNE * DEG2RAD * dt * (drij * kij + drkj * kkj);
+    /* the reference writes the literal 2.51210 here but 143.9325*pi/180 in the gradient
+     * (mmff_bfgs_header.metal:136 vs :154); both are kept */
+    double e = 2.51210 * dt * (drij * kij + drkj * kkj);
     if (g) {
         double s2 = 1.0 - a.cosT * a.cosT;
         if (s2 < 1e-16) return e;
@@ -547,6 +549,57 @@ double oracle_energy_grad(const oracle_mol *m, int family, int diel, const doubl
     }
     return tot;
 }
+
+/* Sum over terms of |term energy|, per term type: the natural scale of FP32 rounding
+ * error in a term-type total (used to set relative tolerances). */
+void oracle_energy_terms_abs(const oracle_mol *m, int family, int diel, const double *x, uint32_t mask,
+                             double *et) {
+    double *g = NULL;
+    double e[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (family == ORACLE_FAMILY_TG) {
+        if (mask & T_BOND)
+            for (int t = 0; t < m->nb; t++) e[0] += fabs(tg_bond(x, g, m->b_i[t], m->b_j[t], m->b_kb[t], m->b_r0[t]));
+        if (mask & T_ANGLE)
+            for (int t = 0; t < m->na; t++)
+                e[1] += fabs(tg_angle(x, g, m->a_i[t], m->a_j[t], m->a_k[t], m->a_ka[t], m->a_t0[t], m->a_lin[t] != 0));
+        if (mask & T_STRBEND)
+            for (int t = 0; t < m->nsb; t++)
+                e[2] += fabs(tg_strbend(x, g, m->s_i[t], m->s_j[t], m->s_k[t], m->s_r0ij[t], m->s_r0kj[t], m->s_t0[t],
+                                   m->s_kij[t], m->s_kkj[t]));
+        if (mask & T_OOP)
+            for (int t = 0; t < m->noop; t++) e[3] += fabs(tg_oop(x, g, m->o_i[t], m->o_j[t], m->o_k[t], m->o_l[t], m->o_koop[t]));
+        if (mask & T_TORSION)
+            for (int t = 0; t < m->nt; t++)
+                e[4] += fabs(tg_torsion(x, g, m->t_i[t], m->t_j[t], m->t_k[t], m->t_l[t], m->t_v1[t], m->t_v2[t], m->t_v3[t]));
+        if (mask & T_VDW)
+            for (int t = 0; t < m->nv; t++) e[5] += fabs(tg_vdw(x, g, m->v_i[t], m->v_j[t], m->v_r[t], m->v_eps[t]));
+        if (mask & T_ELE)
+            for (int t = 0; t < m->ne; t++) e[6] += fabs(tg_ele(x, g, m->e_i[t], m->e_j[t], m->e_qq[t], diel, m->e_14[t] != 0));
+    } else {
+        /* theta0 is stored in degrees; the mega packer converts to radians
+         * (mmff_metal_kernel.py:283-286). */
+        if (mask & T_BOND)
+            for (int t = 0; t < m->nb; t++) e[0] += fabs(mg_bond(x, g, m->b_i[t], m->b_j[t], m->b_kb[t], m->b_r0[t]));
+        if (mask & T_ANGLE)
+            for (int t = 0; t < m->na; t++)
+                e[1] += fabs(mg_angle(x, g, m->a_i[t], m->a_j[t], m->a_k[t], m->a_ka[t], (double)m->a_t0[t] * DEG2RAD));
+        if (mask & T_STRBEND)
+            for (int t = 0; t < m->nsb; t++)
+                e[2] += fabs(mg_strbend(x, g, m->s_i[t], m->s_j[t], m->s_k[t], m->s_kij[t], m->s_kkj[t], m->s_r0ij[t],
+                                   m->s_r0kj[t], (double)m->s_t0[t] * DEG2RAD));
+        if (mask & T_OOP)
+            for (int t = 0; t < m->noop; t++) e[3] += fabs(mg_oop(x, g, m->o_i[t], m->o_j[t], m->o_k[t], m->o_l[t], m->o_koop[t]));
+        if (mask & T_TORSION)
+            for (int t = 0; t < m->nt; t++)
+                e[4] += fabs(mg_torsion(x, g, m->t_i[t], m->t_j[t], m->t_k[t], m->t_l[t], m->t_v1[t], m->t_v2[t], m->t_v3[t]));
+        if (mask & T_VDW)
+            for (int t = 0; t < m->nv; t++) e[5] += fabs(mg_vdw(x, g, m->v_i[t], m->v_j[t], m->v_r[t], m->v_eps[t]));
+        if (mask & T_ELE)
+            for (int t = 0; t < m->ne; t++) e[6] += fabs(mg_ele(x, g, m->e_i[t], m->e_j[t], m->e_qq[t], m->e_14[t] ? 0.75 : 1.0));
+    }
+    for (int k = 0; k < 7; k++) et[k] = e[k];
+}
+
 
 /* ======================= nk optimizers (TG family) ======================= */
 

@@ -44,6 +44,10 @@ enum { T_BOND = 1, T_ANGLE = 2, T_STRBEND = 4, T_OOP = 8, T_TORSION = 16, T_VDW 
 double oracle_energy_grad(const oracle_mol *m, int family, int diel, const double *x,
                           double *g, uint32_t term_mask, double *e_terms);
 
+/* Per-term-type sums of |term energy| (scale of FP32 rounding in each total). */
+void oracle_energy_terms_abs(const oracle_mol *m, int family, int diel, const double *x, uint32_t term_mask,
+                             double *abs7);
+
 /* In-kernel dense BFGS of mlxmolkit/mmff_bfgs_source_tg.metal:1-372.
  * x (3N) in/out. Returns status (0 converged / 1 hit max_iters). */
 int oracle_bfgs_nk(const oracle_mol *m, double *x, int max_iters, double grad_tol, int diel,

@@ -55,6 +55,8 @@ def lib():
         vp = C.c_void_p
         L.oracle_energy_grad.restype = C.c_double
         L.oracle_energy_grad.argtypes = [vp, C.c_int, C.c_int, vp, vp, C.c_uint32, vp]
+        L.oracle_energy_terms_abs.restype = None
+        L.oracle_energy_terms_abs.argtypes = [vp, C.c_int, C.c_int, vp, C.c_uint32, vp]
         L.oracle_bfgs_nk.restype = C.c_int
         L.oracle_bfgs_nk.argtypes = [vp, vp, C.c_int, C.c_double, C.c_int, vp, vp]
         L.oracle_lbfgs_nk.restype = C.c_int
@@ -160,6 +162,15 @@ def energy_grad(params, x, family=FAMILY_TG, diel=1, mask=T_ALL, with_terms=Fals
     if with_terms:
         return e, g, et
     return e, g
+
+
+def terms_abs(params, x, family=FAMILY_TG, diel=1, mask=T_ALL):
+    """Per-term-type sums of |term energy| of one conformer (7 values)."""
+    t = _MolTables(params)
+    xx = np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(-1))
+    out = np.zeros(7)
+    lib().oracle_energy_terms_abs(C.byref(t.mol), family, diel, xx.ctypes.data, mask, out.ctypes.data)
+    return out
 
 
 def energy_grad_batch(batch: OracleBatch, x, family=FAMILY_TG, diel=1, mask=T_ALL, n_threads=None):

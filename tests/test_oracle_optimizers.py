@@ -1,0 +1,154 @@
+"""Pins the oracle's optimizer control flow (BFGS / L-BFGS of the run_mmff kernels, BB
+fused, lock-step multi_mol L-BFGS) with checks that do not restate the code: known
+minima, agreement with scipy's BFGS on the same objective, contract tests restated from
+the reference's own tests (tests/test_metal_multi_mol_convergence.py:39-93,
+tests/test_mmff_minimize_nk.py:25-58) and the closed-form first BB step.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+from scipy.optimize import minimize
+
+import oracle
+from conftest import mol_params
+from mlxmolkit_archive_amd.mmff_params import MMFFParams
+
+
+def one(p, x, **kw):
+    ob = oracle.OracleBatch([p], [1])
+    xo, e, st, cnt = oracle.minimize_nk(ob, np.asarray(x, np.float64), **kw)
+    return xo, e[0], st[0], cnt[0]
+
+
+def test_diatomic_bfgs_reaches_r0():
+    p = MMFFParams(n_atoms=2, bond_idx1=np.array([0], np.int32), bond_idx2=np.array([1], np.int32),
+                   bond_kb=np.array([5.0], np.float32), bond_r0=np.array([1.4], np.float32))
+    for use_lbfgs in (False, True):
+        xo, e, st, cnt = one(p, [0, 0, 0, 1.9, 0.3, -0.2], max_iters=200, grad_tol=1e-8, use_lbfgs=use_lbfgs)
+        assert st == 0
+        assert np.linalg.norm(xo[3:] - xo[:3]) == pytest.approx(np.float32(1.4), abs=2e-5)
+        assert e < 1e-8
+
+
+@pytest.mark.parametrize("smi", ["CCO", "c1ccccc1", "CC(=O)Oc1ccccc1C(=O)O", "CCC#N"])
+def test_bfgs_agrees_with_scipy(smi):
+    """Same objective (FP64 TG energy), scipy BFGS vs the oracle: same local minimum."""
+    m, p, x = mol_params(smi, seed=5)
+    x = x.astype(np.float64)
+    f = lambda z: oracle.energy_grad(p, z)[0]
+    g = lambda z: oracle.energy_grad(p, z)[1]
+    ref = minimize(f, x, jac=g, method="BFGS", options=dict(gtol=1e-7, maxiter=20000))
+    xo, e, st, _ = one(p, x, max_iters=5000, grad_tol=1e-6)
+    assert st == 0
+    assert e == pytest.approx(ref.fun, abs=0.1)
+    assert e <= f(x)
+
+
+def test_max_iters_zero_returns_initial_energy():
+    """tests/test_mmff_minimize_nk.py:25-40: max_iters=0 reports the start energy."""
+    m, p, x = mol_params("c1ccccc1")
+    xo, e, st, cnt = one(p, x, max_iters=0)
+    assert e == pytest.approx(oracle.energy_grad(p, x)[0], rel=1e-12)
+    assert st == 1 and cnt[0] == 0
+    assert np.array_equal(xo, x.astype(np.float64))
+
+
+def test_energy_never_increases_and_status_semantics():
+    m, p, x = mol_params("CCCCCCCCCC", seed=9)
+    e0 = oracle.energy_grad(p, x)[0]
+    for it in (1, 5, 50):
+        xo, e, st, cnt = one(p, x, max_iters=it)
+        assert e <= e0 + 1e-9
+        assert st == 1 and cnt[0] == it          # capped -> status 1
+        assert cnt[2] >= it                      # at least one trial energy per iteration
+    xo, e, st, cnt = one(p, x, max_iters=5000)
+    assert st == 0 and cnt[0] < 5000
+
+
+def test_grad_tol_zero_runs_to_tolx_or_cap():
+    """grad_tol = 0 disables the gradient exit: the run stops only at TOLX / ascent / the cap,
+    so it never stops earlier than with the default tolerance."""
+    m, p, x = mol_params("CCO", seed=2)
+    _, e1, st1, c1 = one(p, x, max_iters=2000, grad_tol=1e-4)
+    _, e0, st0, c0 = one(p, x, max_iters=2000, grad_tol=0.0)
+    assert c0[0] >= c1[0] and e0 <= e1 + 1e-6
+
+
+def test_lbfgs_and_bfgs_reach_the_same_minimum():
+    for smi in ("c1ccccc1Cl", "CC(=O)NC"):
+        m, p, x = mol_params(smi, seed=4)
+        _, e_b, s_b, _ = one(p, x, max_iters=5000, grad_tol=1e-6)
+        _, e_l, s_l, _ = one(p, x, max_iters=5000, grad_tol=1e-6, use_lbfgs=True, m_hist=8)
+        assert s_b == 0 and s_l == 0
+        assert e_l == pytest.approx(e_b, abs=0.05)
+
+
+def test_bb_first_step_closed_form():
+    """First BB iteration: per-atom step min(max_step / |g_a|, grad_lr) along -g_a."""
+    m, p, x = mol_params("CC(=O)O", seed=1)
+    ob = oracle.OracleBatch([p], [1])
+    x1, e1 = oracle.bb_fused(ob, x, max_iters=1, max_step=0.3, grad_lr=0.1)
+    e0, g0 = oracle.energy_grad(p, x, family=oracle.FAMILY_MEGA)
+    g3 = g0.reshape(-1, 3)
+    gn = np.sqrt((g3 ** 2).sum(1) + 1e-30)
+    s = np.minimum(0.3 / gn, 0.1)
+    assert np.allclose(x1.reshape(-1, 3), x.reshape(-1, 3) - s[:, None] * g3, atol=1e-12)
+    assert e1[0] == pytest.approx(e0, rel=1e-12)  # energy of the last pre-step geometry
+    # per-atom displacement never exceeds max_step
+    assert np.sqrt(((x1.reshape(-1, 3) - x.reshape(-1, 3)) ** 2).sum(1)).max() <= 0.3 + 1e-12
+
+
+def test_bb_guard_keeps_energy_bounded():
+    m, p, x = mol_params("CCCCCO", seed=6)
+    ob = oracle.OracleBatch([p], [1])
+    e_start = oracle.energy_grad(p, x, family=oracle.FAMILY_MEGA)[0]
+    es = [oracle.bb_fused(ob, x, max_iters=it)[1][0] for it in (10, 100, 300)]
+    # the guard reverts any step that raises E by more than 1 kcal/mol, so the reported
+    # energy never exceeds the start; it decreases once steps get accepted
+    assert max(es) <= e_start + 1e-9
+    assert es[-1] < e_start - 1.0
+
+
+# ---- multi_mol contract, restated from tests/test_metal_multi_mol_convergence.py:39-93 ----
+
+def _mm_batch(smiles, k=2, seed=42):
+    ps, xs = [], []
+    for i, smi in enumerate(smiles):
+        m, p, x = mol_params(smi, seed=seed + 10 * i, k=k)
+        ps.append(p)
+        xs.append(x)
+    return oracle.OracleBatch(ps, [k] * len(ps)), np.concatenate(xs).astype(np.float64)
+
+
+EASY = ["c1ccccc1", "Clc1ccccc1"]
+HARD = ["CCCCCCCCCCCCCCCC"]
+
+
+def test_multi_mol_per_conf_iters_recorded():
+    ob, x = _mm_batch(EASY)
+    _, e, gn, pci, conv, used = oracle.multi_mol(ob, x, max_iters=2000)
+    assert conv.all()
+    assert (pci > 0).all() and (pci < 500).all()
+    assert np.array_equal(conv, gn <= 1e-4)
+
+
+def test_multi_mol_flexible_needs_more_iterations():
+    ob, x = _mm_batch(EASY + HARD)
+    _, e, gn, pci, conv, used = oracle.multi_mol(ob, x, max_iters=2000)
+    assert pci[-2:].min() > pci[:-2].max()
+
+
+def test_multi_mol_cap_does_not_change_answer():
+    ob, x = _mm_batch(EASY)
+    r1 = oracle.multi_mol(ob, x, max_iters=1000)
+    r4 = oracle.multi_mol(ob, x, max_iters=4000)
+    assert np.allclose(r1[1], r4[1], atol=1e-4)
+    assert np.array_equal(r1[3], r4[3])
+
+
+def test_multi_mol_grad_tol_zero_disables_exit():
+    ob, x = _mm_batch(EASY)
+    _, e, gn, pci, conv, used = oracle.multi_mol(ob, x, max_iters=100, grad_tol=0.0)
+    assert not conv.any()
+    assert (pci == 100).all() and used == 100
