@@ -82,7 +82,8 @@ typedef struct {
   int32_t diel;        /* 1 constant, 2 distance-dependent dielectric */
   int32_t lbfgs_m;     /* history length for MMFFHIP_LBFGS (reference nk kernel uses 8) */
   int32_t hess_place;  /* mmffhip_hess_place */
-  int32_t block;       /* threads per workgroup (64, 128, 192, 256, 320, 512); 0 = plan default */
+  int32_t block;       /* threads per workgroup, a multiple of 64 up to 1024 with an instantiated
+                          kernel for (algo, placement, staging); 0 = plan default */
   int32_t ls_grad;     /* 1: evaluate E+G at every line-search trial (saves the separate
                           gradient pass when the first trial is accepted) */
   const float *precond;  /* optional per-atom 3x3 initial inverse-Hessian blocks, 9 floats per

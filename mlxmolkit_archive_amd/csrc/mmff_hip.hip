@@ -433,6 +433,44 @@ __device__ __forceinline__ double wmax(double v) {
   return fmax(fmax(lane_d(v, 0), lane_d(v, 16)), fmax(lane_d(v, 32), lane_d(v, 48)));
 }
 
+template <int CTRL>
+__device__ __forceinline__ float dpp_f(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
+}
+
+// four consecutive elements (t, j..j+3) of a symmetric matrix in packed upper-triangle
+// storage (column c at packed_floats(c)), with the matching {dg, g} entries
+__device__ __forceinline__ void hrow_load4(const float* H, const float2* VG, int t, int cb, int j, int pj,
+                                           float (&h)[4], float2 (&w)[4]) {
+  const int p1 = pj + j + 1, p2 = p1 + j + 2, p3 = p2 + j + 3;
+  h[0] = H[j <= t ? cb + j : pj + t];
+  h[1] = H[j + 1 <= t ? cb + j + 1 : p1 + t];
+  h[2] = H[j + 2 <= t ? cb + j + 2 : p2 + t];
+  h[3] = H[j + 3 <= t ? cb + j + 3 : p3 + t];
+  w[0] = VG[j];
+  w[1] = VG[j + 1];
+  w[2] = VG[j + 2];
+  w[3] = VG[j + 3];
+}
+
+// Rows [ia, ib) of one column of a packed upper-triangle matrix gain alpha_i s + beta_i h
+// (AB[i] = {alpha_i, beta_i}, u = {s, h} of the column).
+__device__ __forceinline__ void hcol_update(float* col, const float2* AB, float2 u, int ia, int ib) {
+  int i = ia;
+  for (; i + 3 < ib; i += 4) {
+    const float h0 = col[i], h1 = col[i + 1], h2 = col[i + 2], h3 = col[i + 3];
+    const float2 b0 = AB[i], b1 = AB[i + 1], b2 = AB[i + 2], b3 = AB[i + 3];
+    col[i] = fmaf(b0.y, u.y, fmaf(b0.x, u.x, h0));
+    col[i + 1] = fmaf(b1.y, u.y, fmaf(b1.x, u.x, h1));
+    col[i + 2] = fmaf(b2.y, u.y, fmaf(b2.x, u.x, h2));
+    col[i + 3] = fmaf(b3.y, u.y, fmaf(b3.x, u.x, h3));
+  }
+  for (; i < ib; i++) {
+    const float2 b = AB[i];
+    col[i] = fmaf(b.y, u.y, fmaf(b.x, u.x, col[i]));
+  }
+}
+
 template <int NT>
 struct Reducer {
   static constexpr int NW = NT / 64;
@@ -586,8 +624,9 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
   int* s_int = (int*)(red + 16 * NW);
   double* lb_sc = (double*)(s_int + 16);  // rho[16], alpha[16]
   unsigned long long* gacc = (unsigned long long*)(lb_sc + 2 * LBFGS_MAX_M);
-  float4* V = (float4*)(gacc + mn);  // {dg, g, pending xi, pending hd}
-  float* xs = (float*)(V + mn);
+  float2* VG = (float2*)(gacc + mn);  // {dg, g}
+  float2* SH = VG + mn;               // {s, h} of the last rank-2 update (s = x - x_old, h = H dg)
+  float* xs = (float*)(SH + mn);
   float* d = xs + mn;   // direction (also y2 = H g scratch)
   float* xo = d + mn;   // previous x, then xi = x - x_old
   float* hd = xo + mn;  // H dg (BFGS) / two-loop q (L-BFGS)
@@ -615,6 +654,12 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
     const MolDesc m = load_desc(B.mol_desc + 16 * B.conf_mol[c]);
     const int n = 3 * m.n_atoms;
     const int off = B.conf_off[c];
+    // packed-Hessian passes: 2^lsh lanes per row in the read pass, 2^lsh2 lanes per column
+    // pair in the write-back (at most 8 lanes each)
+    const int np = (n + 1) >> 1;
+    int lsh = 0, lsh2 = 0;
+    while (lsh < 3 && (n << (lsh + 1)) <= NT) lsh++;
+    while (lsh2 < 3 && (np << (lsh2 + 1)) <= NT) lsh2++;
     const float* pc = P.precond ? P.precond + (size_t)(off / 3) * 9 : nullptr;
     TView TV;
     if (TLDS) {
@@ -664,7 +709,10 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
         gscale *= 0.5f;
         mxg *= 0.5;
       }
-      for (int i = tid; i < n; i += NT) V[i] = make_float4(0.f, from_fx(gacc[i]) * gscale, 0.f, 0.f);
+      for (int i = tid; i < n; i += NT) {
+        VG[i] = make_float2(0.f, from_fx(gacc[i]) * gscale);
+        SH[i] = make_float2(0.f, 0.f);
+      }
     }
     __syncthreads();
     // initial direction: -g, or -H0 g with the learned preconditioner
@@ -672,10 +720,10 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
       if (pc && ALGO == MMFFHIP_BFGS) {
         const int a = i / 3;
         float sacc = 0.f;
-        for (int q = 0; q < 3; q++) sacc += h_init(pc, i, 3 * a + q) * V[3 * a + q].y;
+        for (int q = 0; q < 3; q++) sacc += h_init(pc, i, 3 * a + q) * VG[3 * a + q].y;
         d[i] = -sacc;
       } else {
-        d[i] = -V[i].y;
+        d[i] = -VG[i].y;
       }
     }
     double ss = 0.0;
@@ -694,7 +742,7 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
       for (int i = tid; i < n; i += NT) {
         xo[i] = xs[i];
         v3[0] += (double)d[i] * d[i];
-        v3[1] += (double)d[i] * V[i].y;
+        v3[1] += (double)d[i] * VG[i].y;
         v3[2] = fmax(v3[2], (double)(fabsf(d[i]) / fmaxf(fabsf(xs[i]), 1.0f)));
       }
       R.template run<3, 4>(v3);
@@ -773,7 +821,7 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
           pc = nullptr;
           pend = false;
           init_h();
-          for (int i = tid; i < n; i += NT) d[i] = -V[i].y;
+          for (int i = tid; i < n; i += NT) d[i] = -VG[i].y;
           __syncthreads();
           continue;
         }
@@ -812,10 +860,7 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
       }
       for (int i = tid; i < n; i += NT) {
         const float gn = from_fx(gacc[i]) * gscale;
-        float4 v = V[i];
-        v.x = gn - v.y;
-        v.y = gn;
-        V[i] = v;
+        VG[i] = make_float2(gn - VG[i].y, gn);
       }
       if (v2[1] / fmax(E * (double)gscale, 1.0) < (double)P.grad_tol) {  // :300-316
         status = 0;
@@ -830,78 +875,73 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
         // factored form U(t, j) = alpha_t s_j + beta_t h_j (2 FMAs per element).
         // Reference: H dg (:318-324), dots (:326-345), rank-2 update (:347-358), d = -H g (:360-366).
         if (HLDS) {
-          // packed upper triangle, column j holds rows 0..j. Phase 1: lane t reads row t of
-          // the columns right of it; phase 2 (after a barrier): lane t owns column t and
-          // writes the updated values back. Loops are unrolled by 4 with independent
-          // accumulators so that four LDS reads are in flight per lane.
-          for (int t = tid; t < n; t += NT) {
-            const float2 ab = AB[t];
-            const float ux = pend ? ab.x : 0.f, uy = pend ? ab.y : 0.f;
-            float a1[4] = {0.f, 0.f, 0.f, 0.f}, a2[4] = {0.f, 0.f, 0.f, 0.f};
-            int j = t + 1;
-            size_t bj = packed_floats(j) + t;
-            for (; j + 3 < n; j += 4) {
-              const size_t b1 = bj + j + 1, b2 = b1 + j + 2, b3 = b2 + j + 3;
-              const float h0 = H[bj], h1 = H[b1], h2 = H[b2], h3 = H[b3];
-              const float4 v0 = V[j], v1 = V[j + 1], v2 = V[j + 2], v3 = V[j + 3];
-              const float g0 = h0 + ux * v0.z + uy * v0.w, g1 = h1 + ux * v1.z + uy * v1.w;
-              const float g2 = h2 + ux * v2.z + uy * v2.w, g3 = h3 + ux * v3.z + uy * v3.w;
-              a1[0] += g0 * v0.x;
-              a2[0] += g0 * v0.y;
-              a1[1] += g1 * v1.x;
-              a2[1] += g1 * v1.y;
-              a1[2] += g2 * v2.x;
-              a2[2] += g2 * v2.y;
-              a1[3] += g3 * v3.x;
-              a2[3] += g3 * v3.y;
-              bj = b3 + j + 4;
-            }
-            for (; j < n; j++) {
-              const float4 vj = V[j];
-              const float h = H[bj] + ux * vj.z + uy * vj.w;
-              a1[0] += h * vj.x;
-              a2[0] += h * vj.y;
-              bj += j + 1;
-            }
-            hd[t] = (a1[0] + a1[1]) + (a1[2] + a1[3]);
-            d[t] = (a2[0] + a2[1]) + (a2[2] + a2[3]);
-          }
-          __syncthreads();
-          for (int t = tid; t < n; t += NT) {
-            const float2 ab = AB[t];
-            const float ux = pend ? ab.x : 0.f, uy = pend ? ab.y : 0.f;
-            float a1[4] = {hd[t], 0.f, 0.f, 0.f}, a2[4] = {d[t], 0.f, 0.f, 0.f};
-            float* col = H + packed_floats(t);
-            int i = 0;
-            for (; i + 3 <= t; i += 4) {
-              const float4 v0 = V[i], v1 = V[i + 1], v2 = V[i + 2], v3 = V[i + 3];
-              const float g0 = col[i] + ux * v0.z + uy * v0.w, g1 = col[i + 1] + ux * v1.z + uy * v1.w;
-              const float g2 = col[i + 2] + ux * v2.z + uy * v2.w, g3 = col[i + 3] + ux * v3.z + uy * v3.w;
-              if (pend) {
-                col[i] = g0;
-                col[i + 1] = g1;
-                col[i + 2] = g2;
-                col[i + 3] = g3;
+          // Packed upper triangle (column j holds rows 0..j), kept fully updated. 2^lsh
+          // consecutive lanes share row t and split its n columns into equal segments;
+          // element (t, j) is read from column t when j <= t and from column j otherwise, so
+          // every lane reads the same number of elements and H is read-only here. The
+          // partial sums of a row are combined with DPP in a fixed order. Loads of the next
+          // four columns are issued before the current four are consumed.
+            const int lpr = 1 << lsh, seg = (n + lpr - 1) >> lsh;
+            for (int r = tid; r < (n << lsh); r += NT) {
+              const int t = r >> lsh, q = r & (lpr - 1);
+              const int j0 = q * seg, j1 = min(n, j0 + seg);
+              const int cb = (int)packed_floats(t);
+              int pj = (int)packed_floats(j0);
+              float a1[4] = {0.f, 0.f, 0.f, 0.f}, a2[4] = {0.f, 0.f, 0.f, 0.f};
+              int j = j0;
+              if (j + 3 < j1) {
+                float h[4];
+                float2 w[4];
+                hrow_load4(H, VG, t, cb, j, pj, h, w);
+                while (j + 7 < j1) {
+                  float hn[4];
+                  float2 wn[4];
+                  const int pn = pj + 4 * j + 10;  // packed_floats(j + 4)
+                  hrow_load4(H, VG, t, cb, j + 4, pn, hn, wn);
+  #pragma unroll
+                  for (int u = 0; u < 4; u++) {
+                    a1[u] += h[u] * w[u].x;
+                    a2[u] += h[u] * w[u].y;
+                    h[u] = hn[u];
+                    w[u] = wn[u];
+                  }
+                  j += 4;
+                  pj = pn;
+                }
+  #pragma unroll
+                for (int u = 0; u < 4; u++) {
+                  a1[u] += h[u] * w[u].x;
+                  a2[u] += h[u] * w[u].y;
+                }
+                pj += 4 * j + 10;
+                j += 4;
               }
-              a1[0] += g0 * v0.x;
-              a2[0] += g0 * v0.y;
-              a1[1] += g1 * v1.x;
-              a2[1] += g1 * v1.y;
-              a1[2] += g2 * v2.x;
-              a2[2] += g2 * v2.y;
-              a1[3] += g3 * v3.x;
-              a2[3] += g3 * v3.y;
+              for (; j < j1; j++) {
+                const float hv = H[j <= t ? cb + j : pj + t];
+                const float2 wj = VG[j];
+                a1[0] += hv * wj.x;
+                a2[0] += hv * wj.y;
+                pj += j + 1;
+              }
+              float s1 = (a1[0] + a1[1]) + (a1[2] + a1[3]), s2 = (a2[0] + a2[1]) + (a2[2] + a2[3]);
+              if (lsh >= 1) {
+                s1 += dpp_f<DPP_QUAD_XOR1>(s1);
+                s2 += dpp_f<DPP_QUAD_XOR1>(s2);
+              }
+              if (lsh >= 2) {
+                s1 += dpp_f<DPP_QUAD_XOR2>(s1);
+                s2 += dpp_f<DPP_QUAD_XOR2>(s2);
+              }
+              if (lsh >= 3) {
+                s1 += dpp_f<DPP_ROW_HALF_MIRROR>(s1);
+                s2 += dpp_f<DPP_ROW_HALF_MIRROR>(s2);
+              }
+              if (q == 0) {
+                hd[t] = s1;
+                d[t] = s2;
+              }
             }
-            for (; i <= t; i++) {
-              const float4 vi = V[i];
-              const float h = col[i] + ux * vi.z + uy * vi.w;
-              if (pend) col[i] = h;
-              a1[0] += h * vi.x;
-              a2[0] += h * vi.y;
-            }
-            hd[t] = (a1[0] + a1[1]) + (a1[2] + a1[3]);
-            d[t] = (a2[0] + a2[1]) + (a2[2] + a2[3]);
-          }
+            __syncthreads();
         } else {
           // full storage, lane t owns column t (= row t by symmetry); coalesced rows,
           // unrolled by 4 so four global reads are in flight per lane
@@ -914,9 +954,10 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
             for (; j + 3 < n; j += 4) {
               float* p0 = col + (size_t)j * n;
               const float h0 = p0[0], h1 = p0[n], h2 = p0[2 * n], h3 = p0[3 * n];
-              const float4 v0 = V[j], v1 = V[j + 1], v2 = V[j + 2], v3 = V[j + 3];
-              const float g0 = h0 + ux * v0.z + uy * v0.w, g1 = h1 + ux * v1.z + uy * v1.w;
-              const float g2 = h2 + ux * v2.z + uy * v2.w, g3 = h3 + ux * v3.z + uy * v3.w;
+              const float2 v0 = VG[j], v1 = VG[j + 1], v2 = VG[j + 2], v3 = VG[j + 3];
+              const float2 u0 = SH[j], u1 = SH[j + 1], u2 = SH[j + 2], u3 = SH[j + 3];
+              const float g0 = h0 + ux * u0.x + uy * u0.y, g1 = h1 + ux * u1.x + uy * u1.y;
+              const float g2 = h2 + ux * u2.x + uy * u2.y, g3 = h3 + ux * u3.x + uy * u3.y;
               if (pend) {
                 p0[0] = g0;
                 p0[n] = g1;
@@ -933,8 +974,8 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
               a2[3] += g3 * v3.y;
             }
             for (; j < n; j++) {
-              const float4 vj = V[j];
-              const float h = col[(size_t)j * n] + ux * vj.z + uy * vj.w;
+              const float2 vj = VG[j], uj = SH[j];
+              const float h = col[(size_t)j * n] + ux * uj.x + uy * uj.y;
               if (pend) col[(size_t)j * n] = h;
               a1[0] += h * vj.x;
               a2[0] += h * vj.y;
@@ -945,7 +986,7 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
         }
         double v6[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         for (int i = tid; i < n; i += NT) {
-          const float4 v = V[i];
+          const float2 v = VG[i];
           const double xi = xo[i], h = hd[i];
           v6[0] += (double)v.x * xi;
           v6[1] += (double)v.x * h;
@@ -966,10 +1007,7 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
             const double xi = xo[t], h = hd[t];
             const double al = A * xi + Bc * h, be = Bc * xi + Cc * h;
             d[t] = -(float)((double)d[t] + al * xig + be * hdg);
-            float4 v = V[t];
-            v.z = (float)xi;
-            v.w = (float)h;
-            V[t] = v;
+            SH[t] = make_float2((float)xi, (float)h);
             AB[t] = make_float2((float)al, (float)be);
           }
           pend = true;
@@ -978,22 +1016,40 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
           pend = false;
         }
         __syncthreads();
+        if (HLDS && pend) {
+          // write the update into the packed triangle: column c gains alpha_i s_c + beta_i h_c
+          // in rows i <= c. Columns c and n-1-c form a pair (n + 1 elements together) and
+          // 2^lsh2 consecutive lanes split a pair's elements evenly; each lane walks its
+          // rows contiguously with s_c, h_c in registers.
+          const int lp = 1 << lsh2;
+          for (int r = tid; r < (np << lsh2); r += NT) {
+            const int g = r >> lsh2, q = r & (lp - 1);
+            const int c1 = g, c2 = n - 1 - g;
+            const bool two = c2 != c1;
+            const int l1 = c1 + 1, lb = l1 + (two ? c2 + 1 : 0);
+            const int sb = (lb + lp - 1) >> lsh2, f0 = q * sb, f1 = min(lb, f0 + sb);
+            if (f0 < l1) hcol_update(H + packed_floats(c1), AB, SH[c1], f0, min(f1, l1));
+            if (two && f1 > l1) hcol_update(H + packed_floats(c2), AB, SH[c2], max(f0, l1) - l1, f1 - l1);
+          }
+          pend = false;
+          __syncthreads();
+        }
       } else {
         // L-BFGS history and two-loop recursion (mmff_lbfgs_source_tg.metal:249-292)
         double ys = 0.0;
-        for (int i = tid; i < n; i += NT) ys += (double)V[i].x * xo[i];
+        for (int i = tid; i < n; i += NT) ys += (double)VG[i].x * xo[i];
         ys = R.sum(ys);
         if (ys > 1e-10) {
           const int slot = hist_idx % M;
           for (int i = tid; i < n; i += NT) {
             S[(size_t)slot * mn + i] = xo[i];
-            Y[(size_t)slot * mn + i] = V[i].x;
+            Y[(size_t)slot * mn + i] = VG[i].x;
           }
           if (tid == 0) rho[slot] = 1.0 / ys;
           hist_idx++;
           if (hist_count < M) hist_count++;
         }
-        for (int i = tid; i < n; i += NT) hd[i] = V[i].y;
+        for (int i = tid; i < n; i += NT) hd[i] = VG[i].y;
         __syncthreads();
         for (int j = hist_count - 1; j >= 0; j--) {
           int slot = (hist_idx - 1 - (hist_count - 1 - j)) % M;
@@ -1437,9 +1493,11 @@ const void* nk_kernel_ptr() {
 
 #define NK_FOR_EACH(X)                                                                                   \
   X(0, true, true, 64) X(0, true, true, 128) X(0, true, true, 192) X(0, true, true, 256)                 \
+  X(0, true, true, 384) X(0, true, true, 512)                                                            \
   X(0, true, false, 64) X(0, true, false, 128) X(0, true, false, 192) X(0, true, false, 256)             \
-  X(0, true, false, 320) X(0, true, false, 512)                                                          \
-  X(0, false, false, 256) X(0, false, false, 320) X(0, false, false, 512)                                \
+  X(0, true, false, 320) X(0, true, false, 384) X(0, true, false, 512) X(0, true, false, 640)            \
+  X(0, true, false, 768) X(0, true, false, 1024)                                                         \
+  X(0, false, false, 256) X(0, false, false, 320) X(0, false, false, 512) X(0, false, false, 1024)       \
   X(1, true, true, 64) X(1, true, true, 128) X(1, true, true, 256)                                       \
   X(1, true, false, 128) X(1, true, false, 256) X(1, true, false, 512)                                   \
   X(1, false, false, 256) X(1, false, false, 512)
@@ -1462,8 +1520,8 @@ int make_nk_plan(const mmffhip_batch* b, const mmffhip_nk_opts* o, NkPlan& pl) {
   DevInfo di = dev_info();
   if (!di.ok) return fail(MMFFHIP_ERR_DEVICE, "no HIP device");
   pl.nt = o->block > 0 ? o->block : (3 * b->max_atoms > 256 ? 512 : 256);
-  if (pl.nt != 64 && pl.nt != 128 && pl.nt != 192 && pl.nt != 256 && pl.nt != 320 && pl.nt != 512)
-    return fail(MMFFHIP_ERR_ARG, "block must be 64, 128, 192, 256, 320 or 512");
+  if (pl.nt < 64 || pl.nt > 1024 || pl.nt % 64 != 0)
+    return fail(MMFFHIP_ERR_ARG, "block must be a multiple of 64 in [64, 1024]");
   pl.max_n = round4(3 * std::max(b->max_atoms, 1));
   const int m = o->algo == MMFFHIP_LBFGS ? o->lbfgs_m : 0;
   const size_t lds_in = nk_lds_bytes(o->algo, true, pl.nt, pl.max_n, m);

@@ -76,7 +76,7 @@ def table_lds_bytes(hb: HostBatch) -> np.ndarray:
 # Each bucket is its own persistent launch whose LDS footprint (and hence occupancy) is
 # sized for its largest molecule. 270 = largest n whose packed FP32 inverse Hessian plus
 # vectors fits the 160 KB of LDS a gfx950 workgroup can own.
-DEFAULT_BUCKETS = ((96, 128, _lib.HESS_AUTO), (192, 192, _lib.HESS_AUTO), (270, 320, _lib.HESS_LDS),
+DEFAULT_BUCKETS = ((96, 256, _lib.HESS_AUTO), (192, 384, _lib.HESS_AUTO), (270, 640, _lib.HESS_LDS),
                    (1 << 20, 512, _lib.HESS_GLOBAL))
 LBFGS_BUCKETS = ((192, 128, _lib.HESS_LDS), (600, 256, _lib.HESS_LDS), (1 << 20, 512, _lib.HESS_AUTO))
 
