@@ -54,7 +54,7 @@ def main():
     ap.add_argument("--max-iters", type=int, default=2000)
     ap.add_argument("--grad-tol", type=float, default=1e-4)
     ap.add_argument("--lbfgs", action="store_true")
-    ap.add_argument("--sample", type=int, default=768, help="conformers checked by the FP64 oracle")
+    ap.add_argument("--sample", type=int, default=4096, help="conformers checked by the FP64 oracle")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -147,6 +147,12 @@ def main():
                        "max_reported_vs_fp64_dE": float(np.max(np.abs(e_gpu[pick] - e_gpu_fp64)))}
         log(f"cpu_baseline {t_cpu:.1f}s; within 0.1 kcal/mol: {within.sum()}/{ok0.sum()} status-0 sampled "
             f"(median |dE| {np.median(dE):.2e})")
+    # measured HBM bytes per step for this workload: rocprofv3 FETCH_SIZE / WRITE_SIZE passes
+    # scaled by the copy-kernel calibration (profiles/k_minimize_nk_medium-1000x10.json)
+    traffic = None
+    prof = ROOT / "profiles" / "k_minimize_nk_medium-1000x10.json"
+    if args.workload == "medium" and args.mols is None and args.confs is None and not args.lbfgs and prof.exists():
+        traffic = float(json.loads(prof.read_text())["traffic_bytes_per_step"])
     conv_est = n_status0 * (frac if frac is not None else 1.0)
     value = conv_est / (kern_ms * 1e-3)
     ok = st == 0
@@ -174,10 +180,14 @@ def main():
         "within_0.1_fraction_sample": None if frac is None else round(frac, 4),
         "wall_ms_per_step": round(wall * 1e3, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK, 4),
+                     "traffic": None if traffic is None else round(traffic),
                      "lds_frac": round(achieved / LDS_PEAK, 4),
-                     "note": "algorithmic bytes (12n^2+40n per BFGS iteration + table/vector bytes per "
-                             "evaluation) over the summed duration of the minimize launches"},
+                     "note": "achieved = algorithmic bytes (12n^2+40n per BFGS iteration + table/vector bytes "
+                             "per evaluation) / summed duration of one step's k_minimize_nk launches; traffic = "
+                             "measured HBM bytes per step (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, copy-"
+                             "calibrated), nearly all from the global-Hessian launch; the Hessian of the other "
+                             "launches lives in LDS"},
         "cpu_baseline": cpu,
         "check": sample_info,
     }
