@@ -113,13 +113,15 @@ __device__ __forceinline__ MolDesc load_desc(const int32_t* md) {
 // is exact for |v| < 2^31, so the accumulated sum equals the exact sum of the FP32
 // contributions truncated to 2^-32, independent of arrival order.
 __device__ __forceinline__ unsigned long long to_fx(float v) {
-  double d = (double)v;
-  if (!(d == d)) d = 0.0;
-  d = fmin(fmax(d, -2147483000.0), 2147483000.0);
-  double h = floor(d);
-  unsigned int lo = (unsigned int)((d - h) * 4294967296.0);
-  int hi = (int)h;
-  return ((unsigned long long)(unsigned int)hi << 32) | (unsigned long long)lo;
+  // v * 2^32 rounded toward zero, as a two's-complement 64-bit integer; so that
+  // to_fx(-v) == -to_fx(v) exactly. The magnitude is split into its integer part and the
+  // fraction |v| - trunc|v|, which FP32 computes exactly (Sterbenz for |v| >= 1);
+  // NaN maps to 0 and magnitudes are capped just below 2^31.
+  const float a = (v == v) ? fminf(fabsf(v), 2147483520.0f) : 0.0f;
+  const float h = truncf(a);
+  const unsigned int lo = (unsigned int)((a - h) * 4294967296.0f);
+  const unsigned long long m = ((unsigned long long)(unsigned int)h << 32) | (unsigned long long)lo;
+  return v < 0.0f ? (unsigned long long)(-(long long)m) : m;
 }
 
 __device__ __forceinline__ float from_fx(unsigned long long a) {
@@ -136,6 +138,18 @@ struct LdsAcc {
       atomicAdd(&g[3 * a], to_fx(x));
       atomicAdd(&g[3 * a + 1], to_fx(y));
       atomicAdd(&g[3 * a + 2], to_fx(z));
+    }
+  }
+  // +g on atom a, -g on atom b: one conversion serves both (to_fx(-x) == -to_fx(x))
+  __device__ __forceinline__ void add_pair(int a, int b, float x, float y, float z) {
+    if (G) {
+      const unsigned long long fx = to_fx(x), fy = to_fx(y), fz = to_fx(z);
+      atomicAdd(&g[3 * a], fx);
+      atomicAdd(&g[3 * a + 1], fy);
+      atomicAdd(&g[3 * a + 2], fz);
+      atomicAdd(&g[3 * b], 0ull - fx);
+      atomicAdd(&g[3 * b + 1], 0ull - fy);
+      atomicAdd(&g[3 * b + 2], 0ull - fz);
     }
   }
 };

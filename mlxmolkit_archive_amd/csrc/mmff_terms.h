@@ -8,9 +8,10 @@
 //               (theta0 converted to radians, CB = -0.007, no linear branch, KEPS = 1e-12,
 //               atan2 torsion, constant dielectric)
 //
-// Gradient contributions go to an accumulator object with add(atom, gx, gy, gz); the
-// kernels use LDS 64-bit fixed-point accumulators so that the sum over terms is exact and
-// independent of the order in which lanes arrive (bit-reproducible runs).
+// Gradient contributions go to an accumulator object with add(atom, gx, gy, gz) and
+// add_pair(a, b, g) (+g on a, -g on b). The kernels use LDS 64-bit fixed-point
+// accumulators, so the sum over terms is exact and independent of the order in which
+// lanes arrive (bit-reproducible runs).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -65,8 +66,7 @@ __device__ __forceinline__ float t_bond(const float* x, Acc& acc, int i, int j, 
       float inv = rg > 1e-8f ? 1.0f / rg : 0.0f;
       float drg = rg - r0;
       float f = MDYNE * kb * drg * (1.0f - 3.0f * drg + (14.0f / 3.0f) * drg * drg) * inv;
-      acc.add(i, f * d.x, f * d.y, f * d.z);
-      acc.add(j, -f * d.x, -f * d.y, -f * d.z);
+      acc.add_pair(i, j, f * d.x, f * d.y, f * d.z);
     }
     return e;
   } else {
@@ -76,8 +76,7 @@ __device__ __forceinline__ float t_bond(const float* x, Acc& acc, int i, int j, 
     if (G) {
       float dE = MDYNE * kb * dr * (1.0f + 1.5f * M_CS * dr + (7.0f / 6.0f) * M_CS * M_CS * dr * dr);
       float f = dE / r;
-      acc.add(i, f * d.x, f * d.y, f * d.z);
-      acc.add(j, -f * d.x, -f * d.y, -f * d.z);
+      acc.add_pair(i, j, f * d.x, f * d.y, f * d.z);
     }
     return e;
   }
@@ -350,8 +349,7 @@ __device__ __forceinline__ void t_pair(const float* x, Acc& acc, int i, int j, f
       float dn1 = diel == 2 ? dpd * dpd * dpd : dpd * dpd;
       float de = -332.0716f * (float)diel * qqs / fmaxf(dn1, 1e-30f);
       float f = (dv + de) * inv;
-      acc.add(i, f * d.x, f * d.y, f * d.z);
-      acc.add(j, -f * d.x, -f * d.y, -f * d.z);
+      acc.add_pair(i, j, f * d.x, f * d.y, f * d.z);
     }
   } else {
     float r = sqrtf(d2) + M_KEPS;
@@ -369,8 +367,7 @@ __device__ __forceinline__ void t_pair(const float* x, Acc& acc, int i, int j, f
       float dv = eps * (7.0f * t17 / t1 * dt1 * t2 + t17 * dt2) / R;
       float de = -M_ELC * qqs / (den * den);
       float f = (dv + de) / r;
-      acc.add(i, f * d.x, f * d.y, f * d.z);
-      acc.add(j, -f * d.x, -f * d.y, -f * d.z);
+      acc.add_pair(i, j, f * d.x, f * d.y, f * d.z);
     }
   }
 }
