@@ -57,6 +57,7 @@ def main():
     ap.add_argument("--sample", type=int, default=4096, help="conformers checked by the FP64 oracle")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a HIP graph")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
@@ -66,10 +67,12 @@ def main():
     import torch
     from mlxmolkit_archive_amd import _lib, engine, workloads
 
-    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    # the workload is built before the GPU is touched: large workloads type their molecules
+    # in forked worker processes
     t0 = time.perf_counter()
     wl = workloads.build(args.workload, n_mols=args.mols, n_confs=args.confs)
     t_gen = time.perf_counter() - t0
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
     log(f"workload {args.workload}: {len(wl.params)} molecules, {wl.n_confs} conformers, "
         f"atoms {wl.atoms_per_conf().min()}-{wl.atoms_per_conf().max()} (built in {t_gen:.1f}s)")
 
@@ -89,15 +92,23 @@ def main():
         runner.run(x_dev, x_out)
         torch.cuda.synchronize()
         log(f"warmup {w + 1}/{args.warmup} done")
+    graph = None
+    if not args.no_graph:
+        graph = runner.capture(x_dev, x_out)
+        log("step captured in a HIP graph (queue resets + one minimize launch per bucket)")
     ev_s = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     ev_e = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for k in range(args.steps):
         ev_s[k].record(stream)
-        res = runner.run(x_dev, x_out)
+        if graph is not None:
+            graph.replay()
+        else:
+            runner.run(x_dev, x_out)
         ev_e[k].record(stream)
     torch.cuda.synchronize()
+    res = engine.NkResult(x_out, runner.e, runner.st, runner.cnt)
     wall = (time.perf_counter() - t_start) / args.steps
     kern_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(ev_s, ev_e)]))
     log(f"timed {args.steps} steps: {wall * 1e3:.1f} ms/step wall, {kern_ms:.1f} ms/step in kernels")
@@ -174,7 +185,8 @@ def main():
                    "algorithm": "L-BFGS m=8" if args.lbfgs else "dense BFGS", "max_iters": args.max_iters,
                    "grad_tol": args.grad_tol, "atoms_min_max": [int(wl.atoms_per_conf().min()),
                                                                 int(wl.atoms_per_conf().max())],
-                   "parallelism": "1 GPU, one workgroup per conformer, persistent size-bucketed launches"},
+                   "parallelism": "1 GPU, one workgroup per conformer, persistent size-bucketed launches",
+                   "launch": "eager" if graph is None else "HIP graph replay"},
         "mean_iters_to_converge": round(mean_iters, 1),
         "status0_fraction": round(n_status0 / wl.n_confs, 4),
         "within_0.1_fraction_sample": None if frac is None else round(frac, 4),

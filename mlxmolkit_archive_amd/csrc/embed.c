@@ -41,19 +41,24 @@ typedef struct {
     const int8_t *rkind;    /* 0 equality, 1 lower bound */
 } embed_mol;
 
-static double restraint_energy_grad(const embed_mol *m, const double *x, double *g, int dim, double w4) {
+/* energy and gradient over the restraints listed in act[0..nact) */
+static double restraint_energy_grad(const embed_mol *m, const int32_t *act, int nact, const double *x, double *g,
+                                    int dim, double w4) {
     int n = m->n_atoms;
     memset(g, 0, sizeof(double) * (size_t)n * dim);
     double e = 0.0;
-    for (int t = 0; t < m->n_restr; t++) {
+    for (int a = 0; a < nact; a++) {
+        const int t = act[a];
         int i = m->ri[t], j = m->rj[t];
         double d2 = 0.0, dv[4];
         for (int c = 0; c < dim; c++) {
             dv[c] = x[i * dim + c] - x[j * dim + c];
             d2 += dv[c] * dv[c];
         }
-        double d = sqrt(d2 + 1e-12);
         double d0 = m->rd[t];
+        /* an inactive lower-bound wall costs no square root */
+        if (m->rkind[t] != 0 && d2 + 1e-12 >= d0 * d0) continue;
+        double d = sqrt(d2 + 1e-12);
         double diff, k;
         if (m->rkind[t] == 0) {
             diff = d - d0;
@@ -80,12 +85,39 @@ static double restraint_energy_grad(const embed_mol *m, const double *x, double 
     return e;
 }
 
+/* Restraints that can act within the next REBUILD steps: every equality spring, and the
+ * lower-bound walls of pairs closer than d_min + 3 A. An atom moves at most 0.3 A per
+ * step (trial steps included), so over REBUILD + 1 = 5 steps a pair closes by at most 3 A
+ * and no wall outside the list can switch on before the next rebuild. */
+#define REBUILD 4
+static int active_restraints(const embed_mol *m, const double *x, int dim, int32_t *act) {
+    int nact = 0;
+    for (int t = 0; t < m->n_restr; t++) {
+        if (m->rkind[t] == 0) {
+            act[nact++] = t;
+            continue;
+        }
+        const int i = m->ri[t], j = m->rj[t];
+        double d2 = 0.0;
+        for (int c = 0; c < dim; c++) {
+            const double v = x[i * dim + c] - x[j * dim + c];
+            d2 += v * v;
+        }
+        const double r = m->rd[t] + 3.0;
+        if (d2 < r * r) act[nact++] = t;
+    }
+    return nact;
+}
+
 /* gradient descent with a per-atom step cap and simple step adaptation */
-static void relax(const embed_mol *m, double *x, double *g, double *xt, double *gt, int dim, double w4, int steps) {
+static void relax(const embed_mol *m, double *x, double *g, double *xt, double *gt, int32_t *act, int dim, double w4,
+                  int steps) {
     int n = m->n_atoms;
     double step = 0.002;
-    double e = restraint_energy_grad(m, x, g, dim, w4);
+    int nact = active_restraints(m, x, dim, act);
+    double e = restraint_energy_grad(m, act, nact, x, g, dim, w4);
     for (int s = 0; s < steps; s++) {
+        if (s > 0 && s % REBUILD == 0) nact = active_restraints(m, x, dim, act);
         for (int i = 0; i < n; i++) {
             double gn = 0.0;
             for (int c = 0; c < dim; c++) gn += g[i * dim + c] * g[i * dim + c];
@@ -94,7 +126,7 @@ static void relax(const embed_mol *m, double *x, double *g, double *xt, double *
             if (gn * sc > 0.3) sc = 0.3 / gn;
             for (int c = 0; c < dim; c++) xt[i * dim + c] = x[i * dim + c] - sc * g[i * dim + c];
         }
-        double et = restraint_energy_grad(m, xt, gt, dim, w4);
+        double et = restraint_energy_grad(m, act, nact, xt, gt, dim, w4);
         if (et <= e) {
             memcpy(x, xt, sizeof(double) * (size_t)n * dim);
             memcpy(g, gt, sizeof(double) * (size_t)n * dim);
@@ -123,6 +155,8 @@ static void *embed_worker(void *arg) {
     embed_job *J = (embed_job *)arg;
     double *buf = NULL;
     size_t cap = 0;
+    int32_t *act = NULL;
+    int act_cap = 0;
     for (;;) {
         pthread_mutex_lock(&J->lock);
         int c = J->next++;
@@ -136,20 +170,26 @@ static void *embed_worker(void *arg) {
             buf = malloc(sizeof(double) * need);
             cap = need;
         }
+        if (m->n_restr > act_cap) {
+            free(act);
+            act = malloc(sizeof(int32_t) * (size_t)m->n_restr);
+            act_cap = m->n_restr;
+        }
         double *x = buf, *g = buf + n * 4, *xt = buf + n * 8, *gt = buf + n * 12;
         rng_t r = {J->seeds[c]};
         double box = 1.5 * cbrt((double)n) + 1.0;
         for (int i = 0; i < n * 4; i++) x[i] = (rng_unif(&r) * 2.0 - 1.0) * box;
-        relax(m, x, g, xt, gt, 4, 0.0, 250);
-        relax(m, x, g, xt, gt, 4, 2.0, 150);
+        relax(m, x, g, xt, gt, act, 4, 0.0, 250);
+        relax(m, x, g, xt, gt, act, 4, 2.0, 150);
         /* collapse to 3D */
         for (int i = 0; i < n; i++)
             for (int k = 0; k < 3; k++) x[i * 3 + k] = x[i * 4 + k];
-        relax(m, x, g, xt, gt, 3, 0.0, 200);
+        relax(m, x, g, xt, gt, act, 3, 0.0, 200);
         float *o = J->out + J->conf_off[c];
         for (int i = 0; i < n * 3; i++) o[i] = (float)(x[i] + (double)J->noise * rng_norm(&r));
     }
     free(buf);
+    free(act);
     return NULL;
 }
 

@@ -85,8 +85,9 @@ class NkRunner:
     """Pre-planned minimize_nk launches for one batch (one launch per size bucket).
 
     ``buckets``: sequence of (max 3N, block, hess_place); conformers go to the first bucket
-    whose bound covers them. ``order``: optional global conformer order (default: longest
-    first inside each bucket).
+    whose bound covers them (``autotune_buckets`` picks them by measurement). ``order``:
+    optional conformer order (default: longest first); conformers left out of it are not
+    minimized.
     """
 
     def __init__(self, db: DeviceBatch, order: Optional[np.ndarray] = None, buckets=None, **opts):
@@ -125,7 +126,7 @@ class NkRunner:
             ordt = torch.from_numpy(np.ascontiguousarray(sel)).to(dev)
             self.launches.append(dict(desc=desc, opts=o, ws=ws, ws_bytes=int(nbytes.value), order=ordt,
                                       n=len(sel), max_n=3 * desc.max_atoms, tbytes=tb))
-        assert assigned.all()
+        assert assigned[order].all(), "a conformer fits no bucket"
         C_ = db.n_confs
         self.e = torch.empty(C_, dtype=torch.float64, device=dev)
         self.st = torch.empty(C_, dtype=torch.int32, device=dev)
@@ -143,6 +144,75 @@ class NkRunner:
                                              _p(self.e), _p(self.st), _p(self.cnt), _p(ln["order"]), _p(ln["ws"]),
                                              ln["ws_bytes"], sp))
         return NkResult(x_out, self.e, self.st, self.cnt)
+
+
+    def capture(self, x_in, x_out):
+        """Record this runner's launches (queue resets and every bucket's minimize kernel) into
+        a HIP graph via torch.cuda.CUDAGraph. ``graph.replay()`` then re-runs the whole
+        minimization of x_in into x_out with one host call; results land in the same tensors
+        as run()'s. The launches clear their work queues with a fill kernel, never a memset,
+        so every replay starts from the same state."""
+        torch = _lib.require_gpu()
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            self.run(x_in, x_out)  # plans, LDS attributes and code objects before capture
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            self.run(x_in, x_out)
+        return g
+
+
+# candidate (block, placement) per size class for the autotuner
+TUNE_CLASSES = ((96, (128, 256, 384)), (192, (192, 384, 512, 768)), (270, (320, 512, 640, 768)),
+                (1 << 20, (256, 512, 1024)))
+
+
+def autotune_buckets(db: DeviceBatch, x, classes=TUNE_CLASSES, sample: int = 512, reps: int = 1, seed: int = 0,
+                     **opts):
+    """Pick the block size (and hence waves per workgroup and workgroups per CU) of each size
+    class by timing candidate launches on a seeded sample of that class's conformers.
+
+    Placement follows the class: LDS for classes whose packed Hessian fits, HBM above.
+    Returns a bucket tuple for NkRunner and the timings {bound: {block: ms}}.
+    """
+    torch = _lib.require_gpu()
+    hb = db.host
+    n3 = 3 * hb.n_atoms[hb.conf_mol]
+    rng = np.random.default_rng(seed)
+    lo = 0
+    buckets, timings = [], {}
+    for bound, blocks in classes:
+        idx = np.nonzero((n3 > lo) & (n3 <= bound))[0]
+        place = _lib.HESS_LDS if bound <= 270 else _lib.HESS_GLOBAL
+        lo = bound
+        if len(idx) == 0:
+            continue
+        pick = np.sort(rng.choice(idx, size=min(sample, len(idx)), replace=False)).astype(np.int32)
+        best, tb = None, {}
+        for blk in blocks:
+            try:
+                r = NkRunner(db, order=pick[np.argsort(-n3[pick], kind="stable")], buckets=((1 << 20, blk, place),),
+                             **opts)
+                r.run(x)
+            except RuntimeError:
+                continue  # the plan does not fit (LDS) or has no kernel: not a candidate
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                r.run(x)
+            e1.record()
+            torch.cuda.synchronize()
+            tb[blk] = e0.elapsed_time(e1) / reps
+            if best is None or tb[blk] < tb[best]:
+                best = blk
+        timings[bound] = tb
+        if best is not None:
+            buckets.append((bound, best, place))
+    return tuple(buckets), timings
 
 
 def minimize_nk(db: DeviceBatch, x, **opts) -> NkResult:

@@ -205,16 +205,14 @@ class _OwnSource:
         return self.t.charges
 
     def vdw_pairs(self, ii, jj):
-        types = np.asarray(self.t.types)
-        R = np.empty(len(ii))
-        E = np.empty(len(ii))
-        cache = {}
-        for k, (i, j) in enumerate(zip(ii, jj)):
-            key = (types[i], types[j])
-            if key not in cache:
-                cache[key] = self.t.vdw(int(i), int(j))[2:]
-            R[k], E[k] = cache[key]
-        return R, E, np.ones(len(ii), dtype=bool)
+        # parameters depend only on the two atom types: evaluate each type pair once
+        types = np.asarray(self.t.types, dtype=np.int64)
+        ii = np.asarray(ii, dtype=np.int64)
+        jj = np.asarray(jj, dtype=np.int64)
+        key = types[ii] * 1024 + types[jj]
+        uk, first, inv = np.unique(key, return_index=True, return_inverse=True)
+        table = np.array([self.t.vdw(int(ii[f]), int(jj[f]))[2:] for f in first], dtype=np.float64).reshape(-1, 2)
+        return table[inv, 0], table[inv, 1], np.ones(len(ii), dtype=bool)
 
 
 def _positions(mol, conf_id: int) -> np.ndarray:

@@ -157,24 +157,19 @@ class Mol:
         return np.array([a.z for a in self.atoms], dtype=np.int32)
 
     def topological_distances(self) -> np.ndarray:
-        """All-pairs shortest path lengths (BFS), like RDKit ``GetDistanceMatrix``."""
+        """All-pairs shortest path lengths in bonds, like RDKit ``GetDistanceMatrix``
+        (unreachable pairs get 10**6)."""
+        from scipy.sparse import csr_matrix
+        from scipy.sparse.csgraph import shortest_path
         n = len(self.atoms)
-        big = 10 ** 6
-        dm = np.full((n, n), big, dtype=np.int64)
-        for s in range(n):
-            dm[s, s] = 0
-            frontier = [s]
-            d = 0
-            while frontier:
-                d += 1
-                nxt = []
-                for u in frontier:
-                    for v in self._nbr[u]:
-                        if dm[s, v] > d:
-                            dm[s, v] = d
-                            nxt.append(v)
-                frontier = nxt
-        return dm
+        if n == 0:
+            return np.zeros((0, 0), dtype=np.int64)
+        rows = [b.begin for b in self.bonds] + [b.end for b in self.bonds]
+        cols = [b.end for b in self.bonds] + [b.begin for b in self.bonds]
+        a = csr_matrix((np.ones(len(rows)), (rows, cols)), shape=(n, n))
+        d = shortest_path(a, method="D", unweighted=True)
+        d[~np.isfinite(d)] = 10 ** 6
+        return d.astype(np.int64)
 
     # --- conformers ------------------------------------------------------
     def GetNumConformers(self) -> int:

@@ -49,6 +49,34 @@ class Workload:
         return np.repeat([p.n_atoms for p in self.params], self.conf_counts)
 
 
+def _typed(args):
+    mols, variant = args
+    out = []
+    for m in mols:
+        # the non-bonded 100 A filter of extract_mmff_params needs a conformer; the
+        # generated molecules are far smaller than 100 A, so a zero geometry keeps every pair
+        m.set_conformers(np.zeros((1, m.GetNumAtoms(), 3)))
+        out.append(extract_mmff_params(m, mmff_variant=variant))
+    return out
+
+
+def _pool_size(n_mols: int) -> int:
+    """Worker processes for typing: only for large batches, and only while no GPU context
+    exists in this process (the workers are forked)."""
+    import os
+    import sys
+    if n_mols < 2000:
+        return 1
+    torch = sys.modules.get("torch")
+    if torch is not None and torch.cuda.is_initialized():
+        return 1
+    try:
+        cpus = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        cpus = os.cpu_count() or 1
+    return max(1, min(16, cpus))
+
+
 def build(name: str, n_mols: Optional[int] = None, n_confs: Optional[int] = None,
           seed: int = molgen.SEED_BASE, noise: float = 0.1, variant: str = "MMFF94",
           n_threads: int = 0) -> Workload:
@@ -56,12 +84,17 @@ def build(name: str, n_mols: Optional[int] = None, n_confs: Optional[int] = None
     M = M if n_mols is None else n_mols
     K = K if n_confs is None else n_confs
     mols = molgen.generate(fam, M, seed=seed)
-    params = []
-    for m in mols:
-        # the non-bonded 100 A filter of extract_mmff_params needs a conformer; the
-        # generated molecules are far smaller than 100 A, so a zero geometry keeps every pair
-        m.set_conformers(np.zeros((1, m.GetNumAtoms(), 3)))
-        params.append(extract_mmff_params(m, mmff_variant=variant))
+    procs = _pool_size(M)
+    if procs > 1:
+        import multiprocessing as mp
+        chunks = [mols[i::procs] for i in range(procs)]
+        with mp.get_context("fork").Pool(procs) as pool:
+            parts = pool.map(_typed, [(c, variant) for c in chunks])
+        params = [None] * M
+        for i, part in enumerate(parts):
+            params[i::procs] = part
+    else:
+        params = _typed((mols, variant))
     counts = [K] * M
     ids = np.arange(M * K, dtype=np.uint64)
     seeds = (np.uint64(seed) * np.uint64(1000003) + np.uint64(molgen._FAMILY_ID[fam]) * np.uint64(7919) * np.uint64(

@@ -258,3 +258,48 @@ def test_empty_batch_is_a_no_op():
     res = r.run(torch.empty(0, dtype=torch.float32, device="cuda"))
     torch.cuda.synchronize()
     assert db.n_confs == 0 and r.launches == [] and res.energy.numel() == 0
+
+
+def test_graph_replay_matches_eager(small_wl):
+    """The captured HIP graph of a runner reproduces the eager run bit for bit, replay after
+    replay, with other eager work and host synchronisations in between."""
+    eng = _eng()
+    wl = small_wl
+    db = eng.device_batch(wl.params, wl.conf_counts)
+    xd = torch.from_numpy(wl.x).cuda()
+    r = eng.NkRunner(db, max_iters=2000, grad_tol=1e-4, ls_grad=True)
+    xo = torch.empty_like(xd)
+    res = r.run(xd, xo)
+    torch.cuda.synchronize()
+    x_ref, e_ref = xo.cpu().numpy().copy(), res.energy.cpu().numpy().copy()
+    st_ref, c_ref = res.status.cpu().numpy().copy(), res.counters.cpu().numpy().copy()
+    g = r.capture(xd, xo)
+    for _ in range(3):
+        xo.zero_()
+        r.e.zero_()
+        r.cnt.zero_()
+        eng.energy_grad(db, xd)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(xo.cpu().numpy(), x_ref)
+        assert np.array_equal(r.e.cpu().numpy(), e_ref)
+        assert np.array_equal(r.st.cpu().numpy(), st_ref)
+        assert np.array_equal(r.cnt.cpu().numpy(), c_ref)
+
+
+def test_autotuned_buckets_cover_batch_and_agree(medium_wl):
+    """autotune_buckets returns a bucket per populated size class whose minima match the
+    default plan's."""
+    eng = _eng()
+    wl = medium_wl
+    db = eng.device_batch(wl.params, wl.conf_counts)
+    xd = torch.from_numpy(wl.x).cuda()
+    buckets, timings = eng.autotune_buckets(db, xd, sample=16, max_iters=2000, grad_tol=1e-4, ls_grad=True)
+    assert buckets and all(len(t) > 0 for t in timings.values())
+    a = eng.NkRunner(db, buckets=buckets, max_iters=2000, grad_tol=1e-4, ls_grad=True).run(xd)
+    b = eng.NkRunner(db, max_iters=2000, grad_tol=1e-4, ls_grad=True).run(xd)
+    torch.cuda.synchronize()
+    ea, eb = a.energy.cpu().numpy(), b.energy.cpu().numpy()
+    assert np.median(np.abs(ea - eb)) < 1e-3
+    assert (a.status.cpu().numpy() == 0).mean() >= 0.9
