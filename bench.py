@@ -139,7 +139,7 @@ def main():
         xgs = np.concatenate([xg[off[c]:off[c] + n3[c]] for c in pick]).astype(np.float64)
         ob = oracle.OracleBatch(sub_params, [1] * S)
         threads = args.cpu_threads or oracle.default_threads()
-        log(f"cpu_baseline: FP64 oracle BFGS on {S} sampled conformers, {threads} threads ...")
+        log(f"cpu_baseline: FP64 oracle on {S} sampled conformers, {threads} threads ...")
         t0 = time.perf_counter()
         _, e_ref, st_ref, cnt_ref = oracle.minimize_nk(ob, xs, max_iters=args.max_iters, grad_tol=args.grad_tol,
                                                        use_lbfgs=args.lbfgs, m_hist=8, n_threads=threads)
@@ -151,8 +151,8 @@ def main():
         frac = float(within.sum() / max(ok0.sum(), 1))
         cpu = {"value": round(float((st_ref == 0).sum() / t_cpu), 3), "unit": "converged conformers/s",
                "cores": int(threads), "kind": "port",
-               "sample": f"{S} seeded conformers of the same workload, FP64 oracle BFGS (same control flow), "
-                         f"{t_cpu:.1f}s wall"}
+               "sample": f"{S} seeded conformers of the same workload, FP64 oracle "
+                         f"{'L-BFGS' if args.lbfgs else 'BFGS'} (same control flow), {t_cpu:.1f}s wall"}
         sample_info = {"sample": S, "status0_in_sample": int(ok0.sum()), "within_0.1_in_sample": int(within.sum()),
                        "median_dE": float(np.median(dE)), "p95_dE": float(np.percentile(dE, 95)),
                        "max_reported_vs_fp64_dE": float(np.max(np.abs(e_gpu[pick] - e_gpu_fp64)))}

@@ -16,6 +16,7 @@ coordinate. Molecule and conformer counts can be scaled down for tests.
 """
 from __future__ import annotations
 
+import sys
 from dataclasses import dataclass
 from typing import List, Optional
 
@@ -99,5 +100,15 @@ def build(name: str, n_mols: Optional[int] = None, n_confs: Optional[int] = None
     ids = np.arange(M * K, dtype=np.uint64)
     seeds = (np.uint64(seed) * np.uint64(1000003) + np.uint64(molgen._FAMILY_ID[fam]) * np.uint64(7919) * np.uint64(
         10 ** 9) + ids)
-    x = embed(params, [m.atomic_numbers() for m in mols], counts, seeds=seeds, noise=noise, n_threads=n_threads)
+    z = [m.atomic_numbers() for m in mols]
+    # large batches are embedded in slices of molecules, with a progress line per slice
+    step = M if M * K <= 20000 else max(1, 20000 // K)
+    parts = []
+    for a in range(0, M, step):
+        b = min(M, a + step)
+        parts.append(embed(params[a:b], z[a:b], counts[a:b], seeds=seeds[a * K:b * K], noise=noise,
+                           n_threads=n_threads))
+        if step < M:
+            print(f"[workloads] {name}: embedded molecules {b}/{M}", file=sys.stderr, flush=True)
+    x = np.concatenate(parts) if len(parts) > 1 else parts[0]
     return Workload(name=name, mols=mols, params=params, conf_counts=counts, x=x)
