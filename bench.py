@@ -58,6 +58,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a HIP graph")
+    ap.add_argument("--precond", action="store_true",
+                    help="start BFGS from the learned per-atom inverse-Hessian blocks (trained first on a "
+                         "separately seeded 300 x 2 training set)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
@@ -76,11 +79,24 @@ def main():
     log(f"workload {args.workload}: {len(wl.params)} molecules, {wl.n_confs} conformers, "
         f"atoms {wl.atoms_per_conf().min()}-{wl.atoms_per_conf().max()} (built in {t_gen:.1f}s)")
 
+    pc_info, pc = None, None
+    if args.precond:
+        from mlxmolkit_archive_amd import molgen, precond
+        t0 = time.perf_counter()
+        tr = workloads.build("medium", n_mols=300, n_confs=2, seed=molgen.SEED_BASE + 1)
+        model, lg = precond.fit(tr)
+        pc = precond.precond_for_batch(model, wl.mols, wl.params, wl.conf_counts)
+        torch.cuda.synchronize()
+        pc_info = {"train": "medium generator, 300 molecules x 2 conformers, seed base + 1, BFGS-minimized",
+                   "target": "diagonal 3x3 blocks of the inverse FD Hessian at the minima",
+                   "loss_first_last": [round(lg.losses[0], 4), round(lg.losses[-1], 4)],
+                   "fit_s": round(time.perf_counter() - t0, 1)}
+        log(f"learned preconditioner fitted in {pc_info['fit_s']}s, loss {pc_info['loss_first_last']}")
     t0 = time.perf_counter()
     db = engine.device_batch(wl.params, wl.conf_counts)
     x_dev = torch.from_numpy(wl.x).cuda()
     runner = engine.NkRunner(db, max_iters=args.max_iters, grad_tol=args.grad_tol, use_lbfgs=args.lbfgs,
-                             ls_grad=True)
+                             ls_grad=True, precond=pc)
     torch.cuda.synchronize()
     t_pack = time.perf_counter() - t0
     log(f"packed + uploaded in {t_pack:.1f}s; launches "
@@ -141,8 +157,12 @@ def main():
         threads = args.cpu_threads or oracle.default_threads()
         log(f"cpu_baseline: FP64 oracle on {S} sampled conformers, {threads} threads ...")
         t0 = time.perf_counter()
+        h0s = None
+        if pc is not None:  # the oracle starts from the same learned blocks
+            pcn = pc.cpu().numpy().astype(np.float64)
+            h0s = np.concatenate([pcn[off[c] // 3:(off[c] + n3[c]) // 3] for c in pick])
         _, e_ref, st_ref, cnt_ref = oracle.minimize_nk(ob, xs, max_iters=args.max_iters, grad_tol=args.grad_tol,
-                                                       use_lbfgs=args.lbfgs, m_hist=8, n_threads=threads)
+                                                       use_lbfgs=args.lbfgs, m_hist=8, n_threads=threads, h0=h0s)
         t_cpu = time.perf_counter() - t0
         e_gpu_fp64, _, _ = oracle.energy_grad_batch(ob, xgs, family=oracle.FAMILY_TG, n_threads=threads)
         dE = np.abs(e_gpu_fp64 - e_ref)
@@ -182,7 +202,8 @@ def main():
         "dtype": "fp32 terms, fp64 energy sums",
         "data": "synthetic: seeded molecules, typed by the built-in MMFF94-style typer, embedded + N(0,0.1A) noise",
         "config": {"workload": f"{args.workload}: {len(wl.params)} molecules x {wl.conf_counts[0]} conformers",
-                   "algorithm": "L-BFGS m=8" if args.lbfgs else "dense BFGS", "max_iters": args.max_iters,
+                   "algorithm": ("L-BFGS m=8" if args.lbfgs else "dense BFGS")
+                                + (" from learned H0" if pc is not None else ""), "max_iters": args.max_iters,
                    "grad_tol": args.grad_tol, "atoms_min_max": [int(wl.atoms_per_conf().min()),
                                                                 int(wl.atoms_per_conf().max())],
                    "parallelism": "1 GPU, one workgroup per conformer, persistent size-bucketed launches",
@@ -201,6 +222,7 @@ def main():
                              "calibrated), nearly all from the global-Hessian launch; the Hessian of the other "
                              "launches lives in LDS"},
         "cpu_baseline": cpu,
+        "precond": pc_info,
         "check": sample_info,
     }
     print(json.dumps(out), flush=True)

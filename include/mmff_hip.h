@@ -87,7 +87,10 @@ typedef struct {
   int32_t ls_grad;     /* 1: evaluate E+G at every line-search trial (saves the separate
                           gradient pass when the first trial is accepted) */
   const float *precond;  /* optional per-atom 3x3 initial inverse-Hessian blocks, 9 floats per
-                            atom at (conf_off/3 + atom)*9; NULL = identity (reference) */
+                            atom at (conf_off/3 + atom)*9, in units of (0.1 * Hessian block)^-1
+                            (the kernel multiplies them by 0.1 / initial gradient scale); BFGS
+                            restarts from the identity if a line search fails. NULL = identity
+                            (reference) */
   int32_t table_lds_bytes; /* >0: stage each molecule's term tables in LDS, reserving this many
                               bytes (ignored when they do not fit beside the Hessian); a
                               conformer whose tables need more gets status 2 and E = NaN */
@@ -129,6 +132,22 @@ int mmffhip_multimol_run(const mmffhip_batch *b, const float *x_in, float *x_bes
                          int32_t max_iters, int32_t eval_freq, float max_step_per_atom, int32_t lbfgs_m,
                          float grad_scale, float grad_tol, void *workspace, size_t workspace_bytes,
                          void *stream);
+
+/* Learned initial inverse Hessian: a per-atom MLP (104 features -> 64 -> 64 -> 6, ReLU) whose 6
+ * outputs are the Cholesky factor of a 3x3 SPD block (diagonal through softplus + 1e-3). No
+ * counterpart in the reference; it produces the optional mmffhip_nk_opts.precond blocks.
+ * params: n_params floats, row-major W1 [64][104], b1, W2 [64][64], b2, W3 [6][64], b3.
+ * feat: n_atoms x 104 floats. blocks: n_atoms x 9 floats (row-major 3x3). */
+int mmffhip_mlp_dims(int32_t *n_in, int32_t *n_hidden, int32_t *n_out, int32_t *n_params);
+/* Inference on MFMA in BF16 with FP32 accumulation. */
+int mmffhip_mlp_infer(const float *params, const float *feat, int32_t n_atoms, float *blocks, void *stream);
+/* Training: FP32 forward and backward of sum_a weight_a * ||block_a - target_a||_F^2.
+ * loss (optional): per-atom unweighted loss; grad (optional): n_params floats, the gradient
+ * summed over atoms in a fixed order (deterministic); weight may be NULL (all ones). */
+int mmffhip_mlp_workspace(int32_t n_atoms, size_t *bytes);
+int mmffhip_mlp_forward_backward(const float *params, const float *feat, int32_t n_atoms, const float *target,
+                                 const float *weight, float *blocks, float *loss, float *grad, void *workspace,
+                                 size_t workspace_bytes, void *stream);
 
 #ifdef __cplusplus
 }

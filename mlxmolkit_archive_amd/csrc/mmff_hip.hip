@@ -622,9 +622,11 @@ __host__ __device__ inline size_t nk_lds_bytes(int algo, bool hlds, int nt, int 
   return b;
 }
 
-// initial inverse Hessian element (identity or the learned per-atom 3x3 block)
-__device__ __forceinline__ float h_init(const float* pc, int i, int j) {
-  if (pc && (i / 3) == (j / 3)) return pc[(j / 3) * 9 + (i % 3) * 3 + (j % 3)];
+// initial inverse Hessian element: the identity, or the learned per-atom 3x3 block times
+// psc (blocks approximate (0.1 * Hessian block)^-1; psc = 0.1 / gradient scale rescales them
+// to the scaled gradient the minimizer works with)
+__device__ __forceinline__ float h_init(const float* pc, float psc, int i, int j) {
+  if (pc) return (i / 3) == (j / 3) ? psc * pc[(j / 3) * 9 + (i % 3) * 3 + (j % 3)] : 0.0f;
   return i == j ? 1.0f : 0.0f;
 }
 
@@ -696,19 +698,19 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
       xs[i] = P.x_in[off + i];
       gacc[i] = 0ull;
     }
+    float psc = 1.0f;
     auto init_h = [&]() {
       if (ALGO != MMFFHIP_BFGS) return;
       if (HLDS) {
         for (int col = tid; col < n; col += NT) {
           float* cp = H + packed_floats(col);
-          for (int i = 0; i <= col; i++) cp[i] = h_init(pc, i, col);
+          for (int i = 0; i <= col; i++) cp[i] = h_init(pc, psc, i, col);
         }
       } else {
         for (int col = tid; col < n; col += NT)
-          for (int j = 0; j < n; j++) H[(size_t)j * n + col] = h_init(pc, j, col);
+          for (int j = 0; j < n; j++) H[(size_t)j * n + col] = h_init(pc, psc, j, col);
       }
     };
-    init_h();
     __syncthreads();
 
     int n_grads = 1, n_evals = 0, iters = 0;
@@ -728,13 +730,15 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
         SH[i] = make_float2(0.f, 0.f);
       }
     }
+    psc = GRAD_SCALE_INIT / gscale;
+    init_h();  // H0 needs the gradient scale when it comes from the learned blocks
     __syncthreads();
     // initial direction: -g, or -H0 g with the learned preconditioner
     for (int i = tid; i < n; i += NT) {
       if (pc && ALGO == MMFFHIP_BFGS) {
         const int a = i / 3;
         float sacc = 0.f;
-        for (int q = 0; q < 3; q++) sacc += h_init(pc, i, 3 * a + q) * VG[3 * a + q].y;
+        for (int q = 0; q < 3; q++) sacc += h_init(pc, psc, i, 3 * a + q) * VG[3 * a + q].y;
         d[i] = -sacc;
       } else {
         d[i] = -VG[i].y;
@@ -1591,6 +1595,9 @@ void launch_nk(int algo, const KBatch& kb, const NkParams& P, const NkPlan& pl, 
 }
 
 }  // namespace
+
+// error reporting shared with the library's other translation units (not exported)
+__attribute__((visibility("hidden"))) int mmffhip_set_error(int code, const char* msg) { return fail(code, msg); }
 
 // ====================================================================================
 // extern "C" ABI

@@ -626,9 +626,10 @@ static void scale_grad(double *g, int n, double *gs) {
  * mmff_bfgs_source_tg.metal:153-256. x_old holds the start point on entry.
  * Returns the accepted energy (or old_e when the search gives up, in which case x = x_old). */
 static double line_search(const oracle_mol *m, int diel, const double *x_old, double *x, const double *d,
-                          double slope, double lam_min, double old_e, int *n_evals) {
+                          double slope, double lam_min, double old_e, int *n_evals, int *accepted) {
     int n = 3 * m->n_atoms;
     double lam = 1.0, prev_lam = 1.0, prev_e = old_e;
+    *accepted = 0;
     for (int ls = 0; ls < MAX_LS_ITERS; ls++) {
         if (lam < lam_min) {
             memcpy(x, x_old, sizeof(double) * (size_t)n);
@@ -637,7 +638,10 @@ static double line_search(const oracle_mol *m, int diel, const double *x_old, do
         for (int i = 0; i < n; i++) x[i] = x_old[i] + lam * d[i];
         double te = oracle_energy_grad(m, ORACLE_FAMILY_TG, diel, x, NULL, T_ALL, NULL);
         (*n_evals)++;
-        if (te - old_e <= FUNCTOL * lam * slope) return te;
+        if (te - old_e <= FUNCTOL * lam * slope) {
+            *accepted = 1;
+            return te;
+        }
         double tmp;
         if (ls == 0) {
             tmp = -slope / (2.0 * (te - old_e - slope));
@@ -677,22 +681,44 @@ typedef struct {
     double *S, *Y, *rho, *alpha;
 } lbfgs_hist;
 
+/* Dense inverse Hessian start: the identity, or (h0 != NULL) the block-diagonal matrix of the
+ * per-atom 3x3 blocks h0 scaled by GRAD_SCALE_INIT / gs -- the learned preconditioner of the
+ * HIP minimizer, which has no counterpart in the reference. */
+static void init_hessian(double *H, int n, const double *h0, double psc) {
+    memset(H, 0, sizeof(double) * (size_t)n * n);
+    if (!h0) {
+        for (int i = 0; i < n; i++) H[(size_t)i * n + i] = 1.0;
+        return;
+    }
+    for (int a = 0; a < n / 3; a++)
+        for (int r = 0; r < 3; r++)
+            for (int c = 0; c < 3; c++) H[(size_t)(3 * a + r) * n + 3 * a + c] = psc * h0[9 * a + 3 * r + c];
+}
+
 static int nk_minimize(const oracle_mol *mol, double *x, int max_iters, double grad_tol, int diel,
-                       lbfgs_hist *hist, double *e_out, int32_t *counters) {
+                       lbfgs_hist *hist, const double *h0, double *e_out, int32_t *counters) {
     int n = 3 * mol->n_atoms;
     size_t nb = sizeof(double) * (size_t)n;
     double *g = malloc(nb), *d = malloc(nb), *xo = malloc(nb), *dg = malloc(nb), *hdg = malloc(nb);
     double *H = NULL;
-    if (!hist) {
-        H = calloc((size_t)n * (size_t)n, sizeof(double));
-        for (int i = 0; i < n; i++) H[(size_t)i * n + i] = 1.0;
-    }
+    if (hist) h0 = NULL;
     int n_evals = 0, n_grads = 1, iters = 0;
     /* initial energy + gradient, :115-135 */
     double energy = oracle_energy_grad(mol, ORACLE_FAMILY_TG, diel, x, g, T_ALL, NULL);
     double gs = GRAD_SCALE_INIT;
     scale_grad(g, n, &gs);
-    for (int i = 0; i < n; i++) d[i] = -g[i];
+    if (!hist) {
+        H = malloc(sizeof(double) * (size_t)n * n);
+        init_hessian(H, n, h0, GRAD_SCALE_INIT / gs);
+    }
+    for (int i = 0; i < n; i++) {
+        double s = 0.0;
+        if (h0)
+            for (int j = 0; j < n; j++) s += H[(size_t)i * n + j] * g[j];
+        else
+            s = g[i];
+        d[i] = -s;
+    }
     double ss = 0.0;
     for (int i = 0; i < n; i++) ss += x[i] * x[i];
     double max_step = MAX_STEP_FACTOR * dmax(sqrt(ss), (double)n); /* :141-145 */
@@ -717,7 +743,15 @@ static int nk_minimize(const oracle_mol *mol, double *x, int max_iters, double g
         double test = 0.0;
         for (int i = 0; i < n; i++) test = dmax(test, fabs(d[i]) / dmax(fabs(x[i]), 1.0));
         double lam_min = MOVETOL / dmax(test, 1e-30);
-        energy = line_search(mol, diel, xo, x, d, slope, lam_min, old_e, &n_evals);
+        int accepted;
+        energy = line_search(mol, diel, xo, x, d, slope, lam_min, old_e, &n_evals, &accepted);
+        if (!accepted && h0) {
+            /* learned start failed: restart from the identity (x is back at x_old) */
+            h0 = NULL;
+            init_hessian(H, n, NULL, 1.0);
+            for (int i = 0; i < n; i++) d[i] = -g[i];
+            continue;
+        }
         /* xi = x - x_old (stored in xo), TOLX :258-279 */
         double tx = 0.0;
         for (int i = 0; i < n; i++) {
@@ -839,7 +873,12 @@ static int nk_minimize(const oracle_mol *mol, double *x, int max_iters, double g
 
 int oracle_bfgs_nk(const oracle_mol *m, double *x, int max_iters, double grad_tol, int diel, double *e_out,
                    int32_t *counters) {
-    return nk_minimize(m, x, max_iters, grad_tol, diel, NULL, e_out, counters);
+    return nk_minimize(m, x, max_iters, grad_tol, diel, NULL, NULL, e_out, counters);
+}
+
+int oracle_bfgs_nk_h0(const oracle_mol *m, double *x, int max_iters, double grad_tol, int diel, const double *h0,
+                      double *e_out, int32_t *counters) {
+    return nk_minimize(m, x, max_iters, grad_tol, diel, NULL, h0, e_out, counters);
 }
 
 int oracle_lbfgs_nk(const oracle_mol *m, double *x, int max_iters, double grad_tol, int diel, int m_hist,
@@ -853,7 +892,7 @@ int oracle_lbfgs_nk(const oracle_mol *m, double *x, int max_iters, double grad_t
     h.Y = calloc((size_t)m_hist * n, sizeof(double));
     h.rho = calloc((size_t)m_hist, sizeof(double));
     h.alpha = calloc((size_t)m_hist, sizeof(double));
-    int st = nk_minimize(m, x, max_iters, grad_tol, diel, &h, e_out, counters);
+    int st = nk_minimize(m, x, max_iters, grad_tol, diel, &h, NULL, e_out, counters);
     free(h.S);
     free(h.Y);
     free(h.rho);
@@ -923,6 +962,7 @@ typedef struct {
     double max_step, grad_lr;
     double *e_out, *g_out, *et_out;
     int32_t *status_out, *counters_out;
+    const double *h0; /* per-atom blocks at (conf_off / 3 + atom) * 9, or NULL */
     int next;
     pthread_mutex_t lock;
 } batch_job;
@@ -940,7 +980,8 @@ static void *batch_worker(void *arg) {
             int32_t *cnt = J->counters_out ? J->counters_out + 3 * (size_t)c : NULL;
             double e;
             int st = J->use_lbfgs ? oracle_lbfgs_nk(m, xc, J->max_iters, J->grad_tol, J->diel, J->m_hist, &e, cnt)
-                                  : oracle_bfgs_nk(m, xc, J->max_iters, J->grad_tol, J->diel, &e, cnt);
+                                  : oracle_bfgs_nk_h0(m, xc, J->max_iters, J->grad_tol, J->diel,
+                                                      J->h0 ? J->h0 + (J->conf_off[c] / 3) * 9 : NULL, &e, cnt);
             J->e_out[c] = e;
             J->status_out[c] = st;
         } else if (J->kind == 1) {
@@ -970,8 +1011,17 @@ static void run_batch(batch_job *J, int n_threads) {
 void oracle_bfgs_nk_batch(const oracle_mol *mols, const int32_t *conf_mol, const int64_t *conf_off, int n_confs,
                           double *x, int max_iters, double grad_tol, int diel, int use_lbfgs, int m_hist,
                           double *e_out, int32_t *status_out, int32_t *counters_out, int n_threads) {
+    oracle_bfgs_nk_batch_h0(mols, conf_mol, conf_off, n_confs, x, max_iters, grad_tol, diel, use_lbfgs, m_hist, NULL,
+                            e_out, status_out, counters_out, n_threads);
+}
+
+void oracle_bfgs_nk_batch_h0(const oracle_mol *mols, const int32_t *conf_mol, const int64_t *conf_off, int n_confs,
+                             double *x, int max_iters, double grad_tol, int diel, int use_lbfgs, int m_hist,
+                             const double *h0, double *e_out, int32_t *status_out, int32_t *counters_out,
+                             int n_threads) {
     batch_job J;
     memset(&J, 0, sizeof J);
+    J.h0 = h0;
     J.kind = 0;
     J.mols = mols;
     J.conf_mol = conf_mol;

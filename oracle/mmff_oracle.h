@@ -53,6 +53,13 @@ void oracle_energy_terms_abs(const oracle_mol *m, int family, int diel, const do
 int oracle_bfgs_nk(const oracle_mol *m, double *x, int max_iters, double grad_tol, int diel,
                    double *e_out, int32_t *counters /* [iters, grads, evals] or NULL */);
 
+/* The same BFGS started from per-atom 3x3 inverse-Hessian blocks h0 (9 doubles per atom) scaled by
+ * 0.1 / initial gradient scale, restarting from the identity if a line search fails: the learned
+ * preconditioner of the HIP minimizer (no counterpart in the reference). h0 == NULL is
+ * oracle_bfgs_nk. */
+int oracle_bfgs_nk_h0(const oracle_mol *m, double *x, int max_iters, double grad_tol, int diel, const double *h0,
+                      double *e_out, int32_t *counters);
+
 /* In-kernel L-BFGS of mlxmolkit/mmff_lbfgs_source_tg.metal:1-298 with history m_hist. */
 int oracle_lbfgs_nk(const oracle_mol *m, double *x, int max_iters, double grad_tol, int diel,
                     int m_hist, double *e_out, int32_t *counters);
@@ -67,6 +74,12 @@ void oracle_bfgs_nk_batch(const oracle_mol *mols, const int32_t *conf_mol, const
                           int n_confs, double *x, int max_iters, double grad_tol, int diel,
                           int use_lbfgs, int m_hist, double *e_out, int32_t *status_out,
                           int32_t *counters_out /* 3 per conf or NULL */, int n_threads);
+
+/* oracle_bfgs_nk_batch with learned blocks h0 at (conf_off[c] / 3 + atom) * 9 (dense BFGS only). */
+void oracle_bfgs_nk_batch_h0(const oracle_mol *mols, const int32_t *conf_mol, const int64_t *conf_off, int n_confs,
+                             double *x, int max_iters, double grad_tol, int diel, int use_lbfgs, int m_hist,
+                             const double *h0, double *e_out, int32_t *status_out, int32_t *counters_out,
+                             int n_threads);
 
 void oracle_energy_grad_batch(const oracle_mol *mols, const int32_t *conf_mol, const int64_t *conf_off,
                               int n_confs, int family, int diel, const double *x, double *e_out,

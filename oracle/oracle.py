@@ -66,6 +66,9 @@ def lib():
         L.oracle_bfgs_nk_batch.restype = None
         L.oracle_bfgs_nk_batch.argtypes = [vp, vp, vp, C.c_int, vp, C.c_int, C.c_double, C.c_int, C.c_int,
                                            C.c_int, vp, vp, vp, C.c_int]
+        L.oracle_bfgs_nk_batch_h0.restype = None
+        L.oracle_bfgs_nk_batch_h0.argtypes = [vp, vp, vp, C.c_int, vp, C.c_int, C.c_double, C.c_int, C.c_int,
+                                              C.c_int, vp, vp, vp, vp, C.c_int]
         L.oracle_energy_grad_batch.restype = None
         L.oracle_energy_grad_batch.argtypes = [vp, vp, vp, C.c_int, C.c_int, C.c_int, vp, vp, vp, C.c_uint32,
                                                vp, C.c_int]
@@ -186,17 +189,22 @@ def energy_grad_batch(batch: OracleBatch, x, family=FAMILY_TG, diel=1, mask=T_AL
 
 
 def minimize_nk(batch: OracleBatch, x, max_iters=500, grad_tol=1e-4, use_lbfgs=False, m_hist=8, diel=1,
-                n_threads=None):
-    """BFGS / L-BFGS of the run_mmff path on every conformer. Returns (x, E, status, counters)."""
+                n_threads=None, h0=None):
+    """BFGS / L-BFGS of the run_mmff path on every conformer. Returns (x, E, status, counters).
+    h0: optional per-atom 3x3 start blocks (total atoms x 9), the learned preconditioner."""
     xx = np.array(x, dtype=np.float64, copy=True).reshape(-1)
     assert xx.size == batch.total
     e = np.zeros(batch.n_confs)
     st = np.zeros(batch.n_confs, dtype=np.int32)
     cnt = np.zeros(3 * batch.n_confs, dtype=np.int32)
-    lib().oracle_bfgs_nk_batch(batch.mols, batch._p(batch.conf_mol), batch._p(batch.conf_off), batch.n_confs,
-                               xx.ctypes.data, int(max_iters), float(grad_tol), diel, int(bool(use_lbfgs)),
-                               int(m_hist), e.ctypes.data, st.ctypes.data, cnt.ctypes.data,
-                               n_threads or default_threads())
+    hh = None
+    if h0 is not None:
+        hh = np.ascontiguousarray(np.asarray(h0, dtype=np.float64).reshape(-1))
+        assert hh.size == 3 * batch.total
+    lib().oracle_bfgs_nk_batch_h0(batch.mols, batch._p(batch.conf_mol), batch._p(batch.conf_off), batch.n_confs,
+                                  xx.ctypes.data, int(max_iters), float(grad_tol), diel, int(bool(use_lbfgs)),
+                                  int(m_hist), None if hh is None else hh.ctypes.data, e.ctypes.data,
+                                  st.ctypes.data, cnt.ctypes.data, n_threads or default_threads())
     return xx, e, st, cnt.reshape(-1, 3)
 
 

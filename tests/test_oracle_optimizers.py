@@ -152,3 +152,28 @@ def test_multi_mol_grad_tol_zero_disables_exit():
     _, e, gn, pci, conv, used = oracle.multi_mol(ob, x, max_iters=100, grad_tol=0.0)
     assert not conv.any()
     assert (pci == 100).all() and used == 100
+
+
+def test_bfgs_h0_identity_blocks_equal_plain_bfgs():
+    """oracle_bfgs_nk_h0 with identity blocks and an unhalved gradient scale is plain BFGS,
+    bit for bit (the block-diagonal start is then exactly the identity)."""
+    m, p, x = mol_params("CCO", noise=0.02)
+    b = oracle.OracleBatch([p], [1])
+    x0 = x.astype(np.float64)
+    e0, g0 = oracle.energy_grad(p, x0)
+    assert np.max(np.abs(0.1 * g0)) <= 10.0  # the gradient scale stays 0.1
+    h0 = np.tile(np.eye(3).reshape(1, 9), (p.n_atoms, 1))
+    xa, ea, sa, ca = oracle.minimize_nk(b, x0, max_iters=300, grad_tol=1e-4)
+    xb, eb, sb, cb = oracle.minimize_nk(b, x0, max_iters=300, grad_tol=1e-4, h0=h0)
+    assert np.array_equal(xa, xb) and np.array_equal(ea, eb) and np.array_equal(ca, cb)
+
+
+def test_bfgs_h0_scaled_start_converges_to_a_minimum():
+    """A start from 0.5 I blocks still converges (status 0) to a point with a small gradient."""
+    m, p, x = mol_params("CC(=O)OC", noise=0.05)
+    b = oracle.OracleBatch([p], [1])
+    h0 = np.tile((0.5 * np.eye(3)).reshape(1, 9), (p.n_atoms, 1))
+    xb, eb, sb, cb = oracle.minimize_nk(b, x.astype(np.float64), max_iters=2000, grad_tol=1e-4, h0=h0)
+    assert sb[0] == 0
+    e, g = oracle.energy_grad(p, xb)
+    assert np.max(np.abs(g)) < 0.1
