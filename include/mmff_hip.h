@@ -38,7 +38,15 @@ enum mmffhip_status {
 
 enum mmffhip_family { MMFFHIP_FAMILY_TG = 0, MMFFHIP_FAMILY_MEGA = 1 };
 enum mmffhip_algo { MMFFHIP_BFGS = 0, MMFFHIP_LBFGS = 1 };
-enum mmffhip_hess_place { MMFFHIP_HESS_AUTO = 0, MMFFHIP_HESS_LDS = 1, MMFFHIP_HESS_GLOBAL = 2 };
+/* where k_minimize_nk keeps the dense inverse Hessian (L-BFGS: its history): AUTO = LDS when it
+ * fits, else HBM; LDS = packed upper triangle in LDS; GLOBAL = full matrix in HBM; GLOBAL_PACKED
+ * = packed upper triangle in HBM (dense BFGS; L-BFGS treats it as GLOBAL) */
+enum mmffhip_hess_place {
+  MMFFHIP_HESS_AUTO = 0,
+  MMFFHIP_HESS_LDS = 1,
+  MMFFHIP_HESS_GLOBAL = 2,
+  MMFFHIP_HESS_GLOBAL_PACKED = 3
+};
 
 /* Device-side ragged batch. Per-molecule term tables are shared by that molecule's
  * conformers; conformer c reads molecule conf_mol[c] and its 3N coordinates start at

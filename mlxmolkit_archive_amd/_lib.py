@@ -18,7 +18,7 @@ FAMILY_TG = 0
 FAMILY_MEGA = 1
 BFGS = 0
 LBFGS = 1
-HESS_AUTO, HESS_LDS, HESS_GLOBAL = 0, 1, 2
+HESS_AUTO, HESS_LDS, HESS_GLOBAL, HESS_GLOBAL_PACKED = 0, 1, 2, 3
 
 _vp = C.c_void_p
 

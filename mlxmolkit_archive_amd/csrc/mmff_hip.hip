@@ -630,8 +630,11 @@ __device__ __forceinline__ float h_init(const float* pc, float psc, int i, int j
   return i == j ? 1.0f : 0.0f;
 }
 
-template <int ALGO, bool HLDS, bool TLDS, int NT>
+// HM: where the dense inverse Hessian lives -- 0 packed in LDS, 1 full in HBM, 2 packed in HBM
+template <int ALGO, int HM, bool TLDS, int NT>
 __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
+  constexpr bool HLDS = HM == 0;    // LDS-resident (also selects the LDS L-BFGS history)
+  constexpr bool PACKED = HM != 1;  // packed upper triangle, select read + column-pair write-back
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int NW = NT / 64;
   const int tid = threadIdx.x;
@@ -701,7 +704,7 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
     float psc = 1.0f;
     auto init_h = [&]() {
       if (ALGO != MMFFHIP_BFGS) return;
-      if (HLDS) {
+      if (PACKED) {
         for (int col = tid; col < n; col += NT) {
           float* cp = H + packed_floats(col);
           for (int i = 0; i <= col; i++) cp[i] = h_init(pc, psc, i, col);
@@ -892,7 +895,7 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
         // The update U = fi s s^T - fei h h^T + fae a a^T (a = fi s - fei h) is kept in the
         // factored form U(t, j) = alpha_t s_j + beta_t h_j (2 FMAs per element).
         // Reference: H dg (:318-324), dots (:326-345), rank-2 update (:347-358), d = -H g (:360-366).
-        if (HLDS) {
+        if (PACKED) {
           // Packed upper triangle (column j holds rows 0..j), kept fully updated. 2^lsh
           // consecutive lanes share row t and split its n columns into equal segments;
           // element (t, j) is read from column t when j <= t and from column j otherwise, so
@@ -961,46 +964,62 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
             }
             __syncthreads();
         } else {
-          // full storage, lane t owns column t (= row t by symmetry); coalesced rows,
-          // unrolled by 4 so four global reads are in flight per lane
-          for (int t = tid; t < n; t += NT) {
+          // full storage in HBM: 2^lsh consecutive lanes share column t (= row t by symmetry)
+          // and split its rows into equal segments; consecutive t are consecutive addresses
+          // (coalesced). The pending update is added as each element is read and written
+          // back by its only owner. Eight loads per lane are issued together.
+          const int lpr = 1 << lsh, seg = (n + lpr - 1) >> lsh;
+          for (int r = tid; r < (n << lsh); r += NT) {
+            const int t = r >> lsh, q = r & (lpr - 1);
+            const int j0 = q * seg, j1 = min(n, j0 + seg);
             const float2 ab = AB[t];
             const float ux = pend ? ab.x : 0.f, uy = pend ? ab.y : 0.f;
             float a1[4] = {0.f, 0.f, 0.f, 0.f}, a2[4] = {0.f, 0.f, 0.f, 0.f};
             float* col = H + t;
-            int j = 0;
-            for (; j + 3 < n; j += 4) {
+            int j = j0;
+            for (; j + 7 < j1; j += 8) {
               float* p0 = col + (size_t)j * n;
-              const float h0 = p0[0], h1 = p0[n], h2 = p0[2 * n], h3 = p0[3 * n];
-              const float2 v0 = VG[j], v1 = VG[j + 1], v2 = VG[j + 2], v3 = VG[j + 3];
-              const float2 u0 = SH[j], u1 = SH[j + 1], u2 = SH[j + 2], u3 = SH[j + 3];
-              const float g0 = h0 + ux * u0.x + uy * u0.y, g1 = h1 + ux * u1.x + uy * u1.y;
-              const float g2 = h2 + ux * u2.x + uy * u2.y, g3 = h3 + ux * u3.x + uy * u3.y;
-              if (pend) {
-                p0[0] = g0;
-                p0[n] = g1;
-                p0[2 * n] = g2;
-                p0[3 * n] = g3;
+              float h[8];
+#pragma unroll
+              for (int u = 0; u < 8; u++) h[u] = p0[(size_t)u * n];
+#pragma unroll
+              for (int u = 0; u < 8; u++) {
+                const float2 v = VG[j + u], w = SH[j + u];
+                h[u] = fmaf(uy, w.y, fmaf(ux, w.x, h[u]));
+                a1[u & 3] = fmaf(h[u], v.x, a1[u & 3]);
+                a2[u & 3] = fmaf(h[u], v.y, a2[u & 3]);
               }
-              a1[0] += g0 * v0.x;
-              a2[0] += g0 * v0.y;
-              a1[1] += g1 * v1.x;
-              a2[1] += g1 * v1.y;
-              a1[2] += g2 * v2.x;
-              a2[2] += g2 * v2.y;
-              a1[3] += g3 * v3.x;
-              a2[3] += g3 * v3.y;
+              if (pend) {
+#pragma unroll
+                for (int u = 0; u < 8; u++) p0[(size_t)u * n] = h[u];
+              }
             }
-            for (; j < n; j++) {
+            for (; j < j1; j++) {
               const float2 vj = VG[j], uj = SH[j];
-              const float h = col[(size_t)j * n] + ux * uj.x + uy * uj.y;
-              if (pend) col[(size_t)j * n] = h;
-              a1[0] += h * vj.x;
-              a2[0] += h * vj.y;
+              const float hv = fmaf(uy, uj.y, fmaf(ux, uj.x, col[(size_t)j * n]));
+              if (pend) col[(size_t)j * n] = hv;
+              a1[0] = fmaf(hv, vj.x, a1[0]);
+              a2[0] = fmaf(hv, vj.y, a2[0]);
             }
-            hd[t] = (a1[0] + a1[1]) + (a1[2] + a1[3]);
-            d[t] = (a2[0] + a2[1]) + (a2[2] + a2[3]);
+            float s1 = (a1[0] + a1[1]) + (a1[2] + a1[3]), s2 = (a2[0] + a2[1]) + (a2[2] + a2[3]);
+            if (lsh >= 1) {
+              s1 += dpp_f<DPP_QUAD_XOR1>(s1);
+              s2 += dpp_f<DPP_QUAD_XOR1>(s2);
+            }
+            if (lsh >= 2) {
+              s1 += dpp_f<DPP_QUAD_XOR2>(s1);
+              s2 += dpp_f<DPP_QUAD_XOR2>(s2);
+            }
+            if (lsh >= 3) {
+              s1 += dpp_f<DPP_ROW_HALF_MIRROR>(s1);
+              s2 += dpp_f<DPP_ROW_HALF_MIRROR>(s2);
+            }
+            if (q == 0) {
+              hd[t] = s1;
+              d[t] = s2;
+            }
           }
+          __syncthreads();
         }
         double v6[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         for (int i = tid; i < n; i += NT) {
@@ -1034,7 +1053,7 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
           pend = false;
         }
         __syncthreads();
-        if (HLDS && pend) {
+        if (PACKED && pend) {
           // write the update into the packed triangle: column c gains alpha_i s_c + beta_i h_c
           // in rows i <= c. Columns c and n-1-c form a pair (n + 1 elements together) and
           // 2^lsh2 consecutive lanes split a pair's elements evenly; each lane walks its
@@ -1495,7 +1514,7 @@ int round4(int v) { return (v + 3) & ~3; }
 
 struct NkPlan {
   int nt = 256;
-  bool hlds = true;
+  int hm = 0;  // Hessian placement of k_minimize_nk (0 packed LDS, 1 full HBM, 2 packed HBM)
   bool tlds = false;
   int tbytes = 0;
   size_t lds = 0;
@@ -1504,25 +1523,27 @@ struct NkPlan {
   int max_n = 0;
 };
 
-template <int ALGO, bool HLDS, bool TLDS, int NT>
+template <int ALGO, int HM, bool TLDS, int NT>
 const void* nk_kernel_ptr() {
-  return (const void*)&k_minimize_nk<ALGO, HLDS, TLDS, NT>;
+  return (const void*)&k_minimize_nk<ALGO, HM, TLDS, NT>;
 }
 
 #define NK_FOR_EACH(X)                                                                                   \
-  X(0, true, true, 64) X(0, true, true, 128) X(0, true, true, 192) X(0, true, true, 256)                 \
-  X(0, true, true, 384) X(0, true, true, 512)                                                            \
-  X(0, true, false, 64) X(0, true, false, 128) X(0, true, false, 192) X(0, true, false, 256)             \
-  X(0, true, false, 320) X(0, true, false, 384) X(0, true, false, 512) X(0, true, false, 640)            \
-  X(0, true, false, 768) X(0, true, false, 1024)                                                         \
-  X(0, false, false, 256) X(0, false, false, 320) X(0, false, false, 512) X(0, false, false, 1024)       \
-  X(1, true, true, 64) X(1, true, true, 128) X(1, true, true, 256)                                       \
-  X(1, true, false, 128) X(1, true, false, 256) X(1, true, false, 512)                                   \
-  X(1, false, false, 256) X(1, false, false, 512)
+  X(0, 0, true, 64) X(0, 0, true, 128) X(0, 0, true, 192) X(0, 0, true, 256)                             \
+  X(0, 0, true, 384) X(0, 0, true, 512)                                                                  \
+  X(0, 0, false, 64) X(0, 0, false, 128) X(0, 0, false, 192) X(0, 0, false, 256)                         \
+  X(0, 0, false, 320) X(0, 0, false, 384) X(0, 0, false, 512) X(0, 0, false, 640)                        \
+  X(0, 0, false, 768) X(0, 0, false, 1024)                                                               \
+  X(0, 1, false, 256) X(0, 1, false, 320) X(0, 1, false, 512) X(0, 1, false, 640)                        \
+  X(0, 1, false, 768) X(0, 1, false, 1024)                                                               \
+  X(0, 2, false, 256) X(0, 2, false, 512) X(0, 2, false, 640) X(0, 2, false, 1024)                       \
+  X(1, 0, true, 64) X(1, 0, true, 128) X(1, 0, true, 256)                                                \
+  X(1, 0, false, 128) X(1, 0, false, 256) X(1, 0, false, 512)                                            \
+  X(1, 1, false, 256) X(1, 1, false, 512)
 
-const void* nk_kernel(int algo, bool hlds, bool tlds, int nt) {
+const void* nk_kernel(int algo, int hm, bool tlds, int nt) {
 #define NKSEL(A, H, TL, T) \
-  if (algo == A && hlds == H && tlds == TL && nt == T) return nk_kernel_ptr<A, H, TL, T>();
+  if (algo == A && hm == H && tlds == TL && nt == T) return nk_kernel_ptr<A, H, TL, T>();
   NK_FOR_EACH(NKSEL)
 #undef NKSEL
   return nullptr;
@@ -1545,24 +1566,28 @@ int make_nk_plan(const mmffhip_batch* b, const mmffhip_nk_opts* o, NkPlan& pl) {
   const size_t lds_in = nk_lds_bytes(o->algo, true, pl.nt, pl.max_n, m);
   const size_t lds_out = nk_lds_bytes(o->algo, false, pl.nt, pl.max_n, m);
   if (o->hess_place == MMFFHIP_HESS_LDS) {
-    pl.hlds = true;
+    pl.hm = 0;
   } else if (o->hess_place == MMFFHIP_HESS_GLOBAL) {
-    pl.hlds = false;
+    pl.hm = 1;
+  } else if (o->hess_place == MMFFHIP_HESS_GLOBAL_PACKED) {
+    pl.hm = o->algo == MMFFHIP_BFGS ? 2 : 1;
+  } else if (o->hess_place == MMFFHIP_HESS_AUTO) {
+    pl.hm = lds_in <= (size_t)di.lds ? 0 : 1;
   } else {
-    pl.hlds = lds_in <= (size_t)di.lds;
+    return fail(MMFFHIP_ERR_ARG, "unknown hess_place");
   }
-  pl.lds = pl.hlds ? lds_in : lds_out;
+  pl.lds = pl.hm == 0 ? lds_in : lds_out;
   if (pl.lds > (size_t)di.lds) return fail(MMFFHIP_ERR_SIZE, "plan exceeds the LDS of one workgroup");
   // stage term tables in LDS when asked and when they fit next to the Hessian
   if (o->table_lds_bytes > 0) {
-    const size_t with = nk_lds_bytes(o->algo, pl.hlds, pl.nt, pl.max_n, m, (size_t)o->table_lds_bytes);
-    if (with <= (size_t)di.lds && nk_kernel(o->algo, pl.hlds, true, pl.nt)) {
+    const size_t with = nk_lds_bytes(o->algo, pl.hm == 0, pl.nt, pl.max_n, m, (size_t)o->table_lds_bytes);
+    if (with <= (size_t)di.lds && nk_kernel(o->algo, pl.hm, true, pl.nt)) {
       pl.tlds = true;
       pl.tbytes = ((o->table_lds_bytes + 15) / 16) * 16;
       pl.lds = with;
     }
   }
-  const void* kp = nk_kernel(o->algo, pl.hlds, pl.tlds, pl.nt);
+  const void* kp = nk_kernel(o->algo, pl.hm, pl.tlds, pl.nt);
   if (!kp) return fail(MMFFHIP_ERR_ARG, "no kernel instantiated for this (algo, placement, block) combination");
   allow_lds(kp, pl.lds);
   int per_cu = 0;
@@ -1570,8 +1595,10 @@ int make_nk_plan(const mmffhip_batch* b, const mmffhip_nk_opts* o, NkPlan& pl) {
     per_cu = 1;
   pl.grid = std::max(1, std::min(b->n_confs, di.n_cu * per_cu));
   pl.slot_floats = 0;
-  if (!pl.hlds) {
-    pl.slot_floats = o->algo == MMFFHIP_LBFGS ? (size_t)2 * m * pl.max_n : (size_t)pl.max_n * pl.max_n;
+  if (pl.hm != 0) {
+    pl.slot_floats = o->algo == MMFFHIP_LBFGS ? (size_t)2 * m * pl.max_n
+                     : pl.hm == 2             ? packed_floats(pl.max_n)
+                                              : (size_t)pl.max_n * pl.max_n;
     pl.slot_floats = (pl.slot_floats + 63) & ~(size_t)63;
   }
   return MMFFHIP_OK;
@@ -1579,14 +1606,14 @@ int make_nk_plan(const mmffhip_batch* b, const mmffhip_nk_opts* o, NkPlan& pl) {
 
 size_t nk_ws_bytes(const NkPlan& pl) { return 256 + (size_t)pl.grid * pl.slot_floats * 4; }
 
-template <int ALGO, bool HLDS, bool TLDS, int NT>
+template <int ALGO, int HM, bool TLDS, int NT>
 void launch_nk_t(const KBatch& kb, const NkParams& P, const NkPlan& pl, hipStream_t s) {
-  hipLaunchKernelGGL((k_minimize_nk<ALGO, HLDS, TLDS, NT>), dim3(pl.grid), dim3(NT), pl.lds, s, kb, P);
+  hipLaunchKernelGGL((k_minimize_nk<ALGO, HM, TLDS, NT>), dim3(pl.grid), dim3(NT), pl.lds, s, kb, P);
 }
 
 void launch_nk(int algo, const KBatch& kb, const NkParams& P, const NkPlan& pl, hipStream_t s) {
 #define NKL(A, H, TL, T)                                          \
-  if (algo == A && pl.hlds == H && pl.tlds == TL && pl.nt == T) { \
+  if (algo == A && pl.hm == H && pl.tlds == TL && pl.nt == T) {   \
     launch_nk_t<A, H, TL, T>(kb, P, pl, s);                       \
     return;                                                       \
   }

@@ -132,14 +132,15 @@ def test_minimize_nk_medium_all_buckets(medium_wl):
 
 
 def test_placements_and_blocks_agree(small_wl):
-    """LDS-packed vs global Hessian, block sizes and LDS table staging: same minima."""
+    """Packed LDS, full HBM and packed HBM Hessians, block sizes and LDS table staging: same minima."""
     eng = _eng()
     wl = small_wl
     db = eng.device_batch(wl.params, wl.conf_counts)
     xd = torch.from_numpy(wl.x).cuda()
     outs = []
     for buckets, stage in ((((1 << 20, 256, 1),), True), (((1 << 20, 256, 2),), False),
-                           (((1 << 20, 64, 1),), True), (((1 << 20, 512, 1),), False)):
+                           (((1 << 20, 64, 1),), True), (((1 << 20, 512, 1),), False),
+                           (((1 << 20, 256, 3),), False)):
         r = eng.NkRunner(db, max_iters=2000, grad_tol=1e-4, buckets=buckets, stage_tables=stage)
         res = r.run(xd)
         torch.cuda.synchronize()
