@@ -222,7 +222,8 @@ def main():
                      "traffic": None if traffic is None else round(traffic),
                      "lds_frac": round(achieved / LDS_PEAK, 4),
                      "note": "achieved = algorithmic bytes (12n^2+40n per BFGS iteration + table/vector bytes "
-                             "per evaluation) / summed duration of one step's k_minimize_nk launches; traffic = "
+                             "per evaluation) of one step / the step's GPU time (HIP events around its "
+                             "k_minimize_nk launches, which overlap on side streams); traffic = "
                              "measured HBM bytes per step (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, copy-"
                              "calibrated), nearly all from the global-Hessian launch; the Hessian of the other "
                              "launches lives in LDS"},

@@ -132,17 +132,38 @@ class NkRunner:
         self.st = torch.empty(C_, dtype=torch.int32, device=dev)
         self.cnt = torch.empty((C_, 3), dtype=torch.int32, device=dev)
 
-    def run(self, x_in, x_out=None, stream=None) -> NkResult:
+    def run(self, x_in, x_out=None, stream=None, overlap: bool = True) -> NkResult:
+        """Minimize every conformer of the plan. With ``overlap`` the size-bucket launches go to
+        up to three side streams (forked from and joined back to ``stream``), so that the
+        HBM-bound large-molecule bucket runs beside the LDS-bound ones and launch tails fill in;
+        the launches write disjoint conformers. Results are the same as sequential launches."""
         torch = _lib.require_gpu()
         assert x_in.dtype == torch.float32 and x_in.numel() == self.db.host.total_coords
         if x_out is None:
             x_out = torch.empty_like(x_in)
         L = _lib.load()
-        sp = _lib.stream_ptr(stream)
-        for ln in self.launches:
+        main = stream if stream is not None else torch.cuda.current_stream()
+        streams = [main]
+        if overlap and len(self.launches) > 1:
+            if not hasattr(self, "_side"):
+                self._side = [torch.cuda.Stream(device=x_in.device) for _ in range(3)]
+            streams = self._side[:min(3, len(self.launches))]
+            for s in streams:
+                s.wait_stream(main)
+        for k, ln in enumerate(self.launches):
+            s = streams[k % len(streams)]
             _lib.check(L.mmffhip_minimize_nk(C.byref(ln["desc"]), C.byref(ln["opts"]), _p(x_in), _p(x_out),
                                              _p(self.e), _p(self.st), _p(self.cnt), _p(ln["order"]), _p(ln["ws"]),
-                                             ln["ws_bytes"], sp))
+                                             ln["ws_bytes"], int(s.cuda_stream)))
+        if streams[0] is not main:
+            for s in streams:
+                main.wait_stream(s)
+            # the tensors are used on the side streams: keep them alive for the caching allocator
+            # (not needed, and not allowed, while a graph is being captured)
+            if not torch.cuda.is_current_stream_capturing():
+                for t in (x_in, x_out, self.e, self.st, self.cnt):
+                    for s in streams:
+                        t.record_stream(s)
         return NkResult(x_out, self.e, self.st, self.cnt)
 
 
