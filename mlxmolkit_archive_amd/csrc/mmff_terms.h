@@ -332,41 +332,49 @@ __device__ __forceinline__ void t_pair(const float* x, Acc& acc, int i, int j, f
   f3 d = ld(x, i) - ld(x, j);
   float d2 = dot(d, d);
   if (FAM == FAM_TG) {
-    float r = sqrtf(fmaxf(d2, 1e-16f));
-    float iR = 1.0f / fmaxf(R, 1e-8f);
-    float q = r * iR;
-    float q2 = q * q, q6 = q2 * q2 * q2, q7 = q6 * q;
-    float t = 1.07f / (q + 0.07f);
-    float t2 = t * t, t7 = t2 * t2 * t2 * t;
-    ev = eps * t7 * (1.12f / (q7 + 0.12f) - 2.0f);
-    float dpd = r + 0.05f;
-    float den = diel == 2 ? dpd * dpd : dpd;
-    ee = 332.0716f * qqs / den;
+    // four reciprocal-type operations per pair: 1/r (r = d2 / r), 1/(q + 0.07),
+    // 1/(q^7 + 0.12) and 1/(r + 0.05), each shared by the energy and the gradient
+    const float d2c = fmaxf(d2, 1e-16f);
+    const float ir = rsqrtf(d2c);
+    const float r = d2c * ir;
+    const float iR = 1.0f / fmaxf(R, 1e-8f);
+    const float q = r * iR;
+    const float q2 = q * q, q6 = q2 * q2 * q2, q7 = q6 * q;
+    const float rq = 1.0f / (q + 0.07f);
+    const float t = 1.07f * rq;
+    const float t2 = t * t, t7 = t2 * t2 * t2 * t;
+    const float r7 = 1.0f / (q7 + 0.12f);  // q >= 0: q7 + 0.12 >= 0.12
+    ev = eps * t7 * (1.12f * r7 - 2.0f);
+    const float dpd = r + 0.05f;
+    const float rd = 1.0f / dpd;  // dpd >= 0.05
+    const float qe = 332.0716f * qqs;
+    ee = diel == 2 ? qe * rd * rd : qe * rd;
     if (G) {
-      float inv = r > 1e-8f ? 1.0f / r : 0.0f;
-      float iq7p = 1.0f / fmaxf(q7 + 0.12f, 1e-30f);
-      float dv = eps * iR * t7 * (-7.84f * q6 * iq7p * iq7p + (-7.84f * iq7p + 14.0f) / (q + 0.07f));
-      float dn1 = diel == 2 ? dpd * dpd * dpd : dpd * dpd;
-      float de = -332.0716f * (float)diel * qqs / fmaxf(dn1, 1e-30f);
-      float f = (dv + de) * inv;
+      const float inv = r > 1e-8f ? ir : 0.0f;
+      const float dv = eps * iR * t7 * (-7.84f * q6 * r7 * r7 + (-7.84f * r7 + 14.0f) * rq);
+      const float de = diel == 2 ? -2.0f * qe * rd * rd * rd : -qe * rd * rd;
+      const float f = (dv + de) * inv;
       acc.add_pair(i, j, f * d.x, f * d.y, f * d.z);
     }
   } else {
-    float r = sqrtf(d2) + M_KEPS;
-    float rho = r / R;
-    float rho7 = rho * rho * rho * rho * rho * rho * rho;
-    float t1 = 1.07f / (rho + 0.07f);
-    float t2 = 1.12f / (rho7 + 0.12f) - 2.0f;
-    float t17 = t1 * t1 * t1 * t1 * t1 * t1 * t1;
+    // shared reciprocals: 1/R, 1/(rho + 0.07), 1/(rho^7 + 0.12), 1/(r + 0.05), 1/r
+    const float r = sqrtf(d2) + M_KEPS;
+    const float iR = 1.0f / R;
+    const float rho = r * iR;
+    const float rho2 = rho * rho, rho6 = rho2 * rho2 * rho2, rho7 = rho6 * rho;
+    const float rq = 1.0f / (rho + 0.07f);
+    const float r7 = 1.0f / (rho7 + 0.12f);
+    const float t1 = 1.07f * rq;
+    const float t2 = 1.12f * r7 - 2.0f;
+    const float t1s = t1 * t1, t16 = t1s * t1s * t1s, t17 = t16 * t1;
     ev = eps * t17 * t2;
-    float den = r + M_ELD;
-    ee = M_ELC * qqs / den;
+    const float rd = 1.0f / (r + M_ELD);
+    ee = M_ELC * qqs * rd;
     if (G) {
-      float dt1 = -1.07f / ((rho + 0.07f) * (rho + 0.07f));
-      float dt2 = -1.12f * 7.0f * rho * rho * rho * rho * rho * rho / ((rho7 + 0.12f) * (rho7 + 0.12f));
-      float dv = eps * (7.0f * t17 / t1 * dt1 * t2 + t17 * dt2) / R;
-      float de = -M_ELC * qqs / (den * den);
-      float f = (dv + de) / r;
+      // d t1 / d rho = -1.07 rq^2 = -t1 rq; d t2 / d rho = -7.84 rho^6 r7^2
+      const float dv = eps * iR * (-7.0f * t17 * rq * t2 - 7.84f * t17 * rho6 * r7 * r7);
+      const float de = -M_ELC * qqs * rd * rd;
+      const float f = (dv + de) / r;
       acc.add_pair(i, j, f * d.x, f * d.y, f * d.z);
     }
   }
