@@ -452,11 +452,11 @@ __device__ __forceinline__ float dpp_f(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xf, 0xf, false));
 }
 
-// four consecutive elements (t, j..j+3) of a symmetric matrix in packed upper-triangle
-// storage (column c at packed_floats(c)), with the matching {dg, g} entries
+// four consecutive elements (t, j..j+3) of a symmetric matrix in padded packed storage
+// (column c at pcol(c), cb = pcol(t), pj = pcol(j)), with the matching {dg, g} entries
 __device__ __forceinline__ void hrow_load4(const float* H, const float2* VG, int t, int cb, int j, int pj,
                                            float (&h)[4], float2 (&w)[4]) {
-  const int p1 = pj + j + 1, p2 = p1 + j + 2, p3 = p2 + j + 3;
+  const int p1 = pj + ((j + 4) & ~3), p2 = p1 + ((j + 5) & ~3), p3 = p2 + ((j + 6) & ~3);
   h[0] = H[j <= t ? cb + j : pj + t];
   h[1] = H[j + 1 <= t ? cb + j + 1 : p1 + t];
   h[2] = H[j + 2 <= t ? cb + j + 2 : p2 + t];
@@ -467,21 +467,20 @@ __device__ __forceinline__ void hrow_load4(const float* H, const float2* VG, int
   w[3] = VG[j + 3];
 }
 
-// Rows [ia, ib) of one column of a packed upper-triangle matrix gain alpha_i s + beta_i h
-// (AB[i] = {alpha_i, beta_i}, u = {s, h} of the column).
-__device__ __forceinline__ void hcol_update(float* col, const float2* AB, float2 u, int ia, int ib) {
-  int i = ia;
-  for (; i + 3 < ib; i += 4) {
-    const float h0 = col[i], h1 = col[i + 1], h2 = col[i + 2], h3 = col[i + 3];
-    const float2 b0 = AB[i], b1 = AB[i + 1], b2 = AB[i + 2], b3 = AB[i + 3];
-    col[i] = fmaf(b0.y, u.y, fmaf(b0.x, u.x, h0));
-    col[i + 1] = fmaf(b1.y, u.y, fmaf(b1.x, u.x, h1));
-    col[i + 2] = fmaf(b2.y, u.y, fmaf(b2.x, u.x, h2));
-    col[i + 3] = fmaf(b3.y, u.y, fmaf(b3.x, u.x, h3));
-  }
-  for (; i < ib; i++) {
-    const float2 b = AB[i];
-    col[i] = fmaf(b.y, u.y, fmaf(b.x, u.x, col[i]));
+// Row groups [ka, kb) (rows 4k..4k+3) of one column of the padded packed matrix gain
+// alpha_i s + beta_i h (AB[i] = {alpha_i, beta_i}, u = {s, h} of the column); rows past the
+// column's end are padding and never read as matrix elements.
+__device__ __forceinline__ void hcol_update4(float* col, const float2* AB, float2 u, int ka, int kb) {
+  float4* c4 = (float4*)col;
+  const float4* ab4 = (const float4*)AB;
+  for (int k = ka; k < kb; k++) {
+    float4 h = c4[k];
+    const float4 a01 = ab4[2 * k], a23 = ab4[2 * k + 1];
+    h.x = fmaf(a01.y, u.y, fmaf(a01.x, u.x, h.x));
+    h.y = fmaf(a01.w, u.y, fmaf(a01.z, u.x, h.y));
+    h.z = fmaf(a23.y, u.y, fmaf(a23.x, u.x, h.z));
+    h.w = fmaf(a23.w, u.y, fmaf(a23.z, u.x, h.w));
+    c4[k] = h;
   }
 }
 
@@ -603,7 +602,15 @@ struct NkParams {
   int tbytes;         // LDS bytes reserved for staged term tables (TLDS kernels)
 };
 
-__host__ __device__ inline size_t packed_floats(int n) { return (size_t)n * (n + 1) / 2; }
+// Packed upper-triangle storage of a symmetric n x n matrix with every column padded to a
+// multiple of 4 floats (16 bytes): column c holds rows 0..c from offset pcol(c) =
+// sum_{k<c} round4(k + 1), so every column starts 16-byte aligned and can be walked in float4
+// steps. packed_floats(n) = pcol(n) floats hold the matrix.
+__host__ __device__ inline int pcol(int c) {
+  const int m = c >> 2, r = c & 3;
+  return 8 * m * (m + 1) + 4 * r * (m + 1);
+}
+__host__ __device__ inline size_t packed_floats(int n) { return (size_t)pcol(n); }
 
 // LDS footprint (bytes) of the nk kernel
 __host__ __device__ inline size_t nk_lds_bytes(int algo, bool hlds, int nt, int max_n, int m, size_t tbytes = 0) {
@@ -653,7 +660,7 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
   float* big = (float*)(AB + mn);
   float* H = HLDS ? big : P.hglob + (size_t)blockIdx.x * P.slot_floats;
   size_t big_floats = 0;
-  if (HLDS) big_floats = ALGO == MMFFHIP_BFGS ? ((packed_floats(mn) + 3) & ~(size_t)3) : 2 * (size_t)P.m * mn;
+  if (HLDS) big_floats = ALGO == MMFFHIP_BFGS ? packed_floats(mn) : 2 * (size_t)P.m * mn;
   unsigned char* tstage = (unsigned char*)(big + big_floats);
   const int M = P.m;
   float* S = H;
@@ -706,8 +713,8 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
       if (ALGO != MMFFHIP_BFGS) return;
       if (PACKED) {
         for (int col = tid; col < n; col += NT) {
-          float* cp = H + packed_floats(col);
-          for (int i = 0; i <= col; i++) cp[i] = h_init(pc, psc, i, col);
+          float* cp = H + pcol(col);
+          for (int i = 0; i < ((col + 4) & ~3); i++) cp[i] = i <= col ? h_init(pc, psc, i, col) : 0.0f;
         }
       } else {
         for (int col = tid; col < n; col += NT)
@@ -906,8 +913,8 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
             for (int r = tid; r < (n << lsh); r += NT) {
               const int t = r >> lsh, q = r & (lpr - 1);
               const int j0 = q * seg, j1 = min(n, j0 + seg);
-              const int cb = (int)packed_floats(t);
-              int pj = (int)packed_floats(j0);
+              const int cb = pcol(t);
+              int pj = pcol(j0);
               float a1[4] = {0.f, 0.f, 0.f, 0.f}, a2[4] = {0.f, 0.f, 0.f, 0.f};
               int j = j0;
               if (j + 3 < j1) {
@@ -917,7 +924,7 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
                 while (j + 7 < j1) {
                   float hn[4];
                   float2 wn[4];
-                  const int pn = pj + 4 * j + 10;  // packed_floats(j + 4)
+                  const int pn = pcol(j + 4);
                   hrow_load4(H, VG, t, cb, j + 4, pn, hn, wn);
   #pragma unroll
                   for (int u = 0; u < 4; u++) {
@@ -934,15 +941,15 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
                   a1[u] += h[u] * w[u].x;
                   a2[u] += h[u] * w[u].y;
                 }
-                pj += 4 * j + 10;
                 j += 4;
+                pj = pcol(j);
               }
               for (; j < j1; j++) {
                 const float hv = H[j <= t ? cb + j : pj + t];
                 const float2 wj = VG[j];
                 a1[0] += hv * wj.x;
                 a2[0] += hv * wj.y;
-                pj += j + 1;
+                pj += (j + 4) & ~3;
               }
               float s1 = (a1[0] + a1[1]) + (a1[2] + a1[3]), s2 = (a2[0] + a2[1]) + (a2[2] + a2[3]);
               if (lsh >= 1) {
@@ -1055,18 +1062,18 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
         __syncthreads();
         if (PACKED && pend) {
           // write the update into the packed triangle: column c gains alpha_i s_c + beta_i h_c
-          // in rows i <= c. Columns c and n-1-c form a pair (n + 1 elements together) and
-          // 2^lsh2 consecutive lanes split a pair's elements evenly; each lane walks its
-          // rows contiguously with s_c, h_c in registers.
+          // in rows i <= c. Columns c and n-1-c form a pair and 2^lsh2 consecutive lanes split
+          // the pair's 4-row groups evenly; each lane walks its groups in float4 steps with
+          // s_c, h_c in registers (columns are 16-byte aligned).
           const int lp = 1 << lsh2;
           for (int r = tid; r < (np << lsh2); r += NT) {
             const int g = r >> lsh2, q = r & (lp - 1);
             const int c1 = g, c2 = n - 1 - g;
             const bool two = c2 != c1;
-            const int l1 = c1 + 1, lb = l1 + (two ? c2 + 1 : 0);
+            const int l1 = (c1 + 4) >> 2, lb = l1 + (two ? (c2 + 4) >> 2 : 0);  // 4-row groups
             const int sb = (lb + lp - 1) >> lsh2, f0 = q * sb, f1 = min(lb, f0 + sb);
-            if (f0 < l1) hcol_update(H + packed_floats(c1), AB, SH[c1], f0, min(f1, l1));
-            if (two && f1 > l1) hcol_update(H + packed_floats(c2), AB, SH[c2], max(f0, l1) - l1, f1 - l1);
+            if (f0 < l1) hcol_update4(H + pcol(c1), AB, SH[c1], f0, min(f1, l1));
+            if (two && f1 > l1) hcol_update4(H + pcol(c2), AB, SH[c2], max(f0, l1) - l1, f1 - l1);
           }
           pend = false;
           __syncthreads();

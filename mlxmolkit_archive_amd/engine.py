@@ -74,9 +74,9 @@ def table_lds_bytes(hb: HostBatch) -> np.ndarray:
 
 # Size buckets for the nk minimizer: (max 3N, threads per workgroup, Hessian placement).
 # Each bucket is its own persistent launch whose LDS footprint (and hence occupancy) is
-# sized for its largest molecule. 270 = largest n whose packed FP32 inverse Hessian plus
+# sized for its largest molecule. 264 = largest n whose packed FP32 inverse Hessian plus
 # vectors fits the 160 KB of LDS a gfx950 workgroup can own.
-DEFAULT_BUCKETS = ((96, 256, _lib.HESS_AUTO), (192, 384, _lib.HESS_AUTO), (270, 640, _lib.HESS_LDS),
+DEFAULT_BUCKETS = ((96, 256, _lib.HESS_AUTO), (192, 384, _lib.HESS_AUTO), (264, 640, _lib.HESS_LDS),
                    (1 << 20, 512, _lib.HESS_GLOBAL))
 LBFGS_BUCKETS = ((192, 128, _lib.HESS_LDS), (600, 256, _lib.HESS_LDS), (1 << 20, 512, _lib.HESS_AUTO))
 
@@ -187,7 +187,7 @@ class NkRunner:
 
 
 # candidate (block, placement) per size class for the autotuner
-TUNE_CLASSES = ((96, (128, 256, 384)), (192, (192, 384, 512, 768)), (270, (320, 512, 640, 768)),
+TUNE_CLASSES = ((96, (128, 256, 384)), (192, (192, 384, 512, 768)), (264, (320, 512, 640, 768)),
                 (1 << 20, (256, 512, 1024)))
 
 
@@ -207,7 +207,7 @@ def autotune_buckets(db: DeviceBatch, x, classes=TUNE_CLASSES, sample: int = 512
     buckets, timings = [], {}
     for bound, blocks in classes:
         idx = np.nonzero((n3 > lo) & (n3 <= bound))[0]
-        place = _lib.HESS_LDS if bound <= 270 else _lib.HESS_GLOBAL
+        place = _lib.HESS_LDS if bound <= 264 else _lib.HESS_GLOBAL
         lo = bound
         if len(idx) == 0:
             continue
