@@ -26,7 +26,6 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK = 8.0e12       # B/s, MI355X
-LDS_PEAK = 256 * 128 * 2.4e9  # B/s, 256 CUs x 128 B/clk
 
 
 def log(msg):
@@ -220,13 +219,15 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK, 4),
                      "traffic": None if traffic is None else round(traffic),
-                     "lds_frac": round(achieved / LDS_PEAK, 4),
+                     "traffic_frac": None if traffic is None else round(traffic / (kern_ms * 1e-3) / HBM_PEAK, 4),
                      "note": "achieved = algorithmic bytes (12n^2+40n per BFGS iteration + table/vector bytes "
                              "per evaluation) of one step / the step's GPU time (HIP events around its "
-                             "k_minimize_nk launches, which overlap on side streams); traffic = "
-                             "measured HBM bytes per step (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, copy-"
-                             "calibrated), nearly all from the global-Hessian launch; the Hessian of the other "
-                             "launches lives in LDS"},
+                             "k_minimize_nk launches); it exceeds the HBM peak because each conformer's "
+                             "Hessian stays on chip: in LDS up to n = 96, above that in L2 and the 256 MB "
+                             "Infinity Cache (one slot per resident workgroup). traffic = measured bytes per "
+                             "step between L2 and memory (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, copy-"
+                             "calibrated; Infinity Cache hits count as traffic); traffic_frac = traffic / "
+                             "step time / HBM peak"},
         "cpu_baseline": cpu,
         "precond": pc_info,
         "check": sample_info,

@@ -121,7 +121,8 @@ int mmffhip_minimize_nk_workspace(const mmffhip_batch *b, const mmffhip_nk_opts 
 /* In-kernel BFGS / L-BFGS minimization of every conformer (reference run_mmff semantics).
  * x_in / x_out may alias. order: conformer processing order (length n_confs, e.g. longest
  * first), NULL = natural order. counters (optional): 3 int32 per conformer
- * {iterations, gradient evaluations, energy evaluations}. */
+ * {iterations, gradient evaluations, energy evaluations}. workspace: 256-byte aligned device
+ * memory of at least the queried size (hipMalloc and PyTorch allocations are). */
 int mmffhip_minimize_nk(const mmffhip_batch *b, const mmffhip_nk_opts *o, const float *x_in, float *x_out,
                         double *e_out, int32_t *status_out, int32_t *counters, const int32_t *order,
                         void *workspace, size_t workspace_bytes, void *stream);

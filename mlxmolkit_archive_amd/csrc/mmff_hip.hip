@@ -2,9 +2,9 @@
 //
 // Execution model (one conformer per workgroup, persistent grid with a work queue):
 //   * positions, gradient accumulators and BFGS/L-BFGS vectors live in LDS;
-//   * the dense inverse Hessian lives in LDS when it fits (n = 3N up to ~195 at one
-//     workgroup per CU), otherwise in a per-workgroup global slot (L2/MALL resident),
-//     stored column-per-thread so every H pass is conflict-free / coalesced;
+//   * the dense inverse Hessian lives in LDS as a packed triangle with 16-byte aligned
+//     columns when it fits (n = 3N up to 264 at one workgroup per CU), otherwise in full
+//     storage in a per-workgroup global slot, read as coalesced 16-byte column quads;
 //   * term energies and gradients are FP32; each lane's energy is accumulated in FP64 and
 //     reduced with wave shuffles + one LDS exchange, so the Armijo test and reported
 //     energies are FP64 sums; gradients are summed exactly in 64-bit fixed point with LDS
@@ -467,6 +467,24 @@ __device__ __forceinline__ void hrow_load4(const float* H, const float2* VG, int
   w[3] = VG[j + 3];
 }
 
+// one row of a column quad of the full Hessian: add the pending update (ux, uy per column,
+// w = {s, h} of the row) and accumulate the row's {dg, g} = v into the quad's two dot products
+__device__ __forceinline__ void hquad_step(float4& h, float2 v, float2 w, const float (&ux)[4],
+                                           const float (&uy)[4], float (&a1)[4], float (&a2)[4]) {
+  h.x = fmaf(uy[0], w.y, fmaf(ux[0], w.x, h.x));
+  h.y = fmaf(uy[1], w.y, fmaf(ux[1], w.x, h.y));
+  h.z = fmaf(uy[2], w.y, fmaf(ux[2], w.x, h.z));
+  h.w = fmaf(uy[3], w.y, fmaf(ux[3], w.x, h.w));
+  a1[0] = fmaf(h.x, v.x, a1[0]);
+  a2[0] = fmaf(h.x, v.y, a2[0]);
+  a1[1] = fmaf(h.y, v.x, a1[1]);
+  a2[1] = fmaf(h.y, v.y, a2[1]);
+  a1[2] = fmaf(h.z, v.x, a1[2]);
+  a2[2] = fmaf(h.z, v.y, a2[2]);
+  a1[3] = fmaf(h.w, v.x, a1[3]);
+  a2[3] = fmaf(h.w, v.y, a2[3]);
+}
+
 // Row groups [ka, kb) (rows 4k..4k+3) of one column of the padded packed matrix gain
 // alpha_i s + beta_i h (AB[i] = {alpha_i, beta_i}, u = {s, h} of the column); rows past the
 // column's end are padding and never read as matrix elements.
@@ -717,8 +735,9 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
           for (int i = 0; i < ((col + 4) & ~3); i++) cp[i] = i <= col ? h_init(pc, psc, i, col) : 0.0f;
         }
       } else {
-        for (int col = tid; col < n; col += NT)
-          for (int j = 0; j < n; j++) H[(size_t)j * n + col] = h_init(pc, psc, j, col);
+        const int ldh = (n + 3) & ~3;  // rows padded to 16 bytes; padding columns stay zero
+        for (int col = tid; col < ldh; col += NT)
+          for (int j = 0; j < n; j++) H[(size_t)j * ldh + col] = col < n ? h_init(pc, psc, j, col) : 0.0f;
       }
     };
     __syncthreads();
@@ -971,59 +990,70 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
             }
             __syncthreads();
         } else {
-          // full storage in HBM: 2^lsh consecutive lanes share column t (= row t by symmetry)
-          // and split its rows into equal segments; consecutive t are consecutive addresses
-          // (coalesced). The pending update is added as each element is read and written
-          // back by its only owner. Eight loads per lane are issued together.
-          const int lpr = 1 << lsh, seg = (n + lpr - 1) >> lsh;
-          for (int r = tid; r < (n << lsh); r += NT) {
-            const int t = r >> lsh, q = r & (lpr - 1);
+          // full storage in HBM with rows padded to ldh = round4(n) floats (16-byte aligned).
+          // 2^lq consecutive lanes share the column quad 4tq..4tq+3 and split its rows into
+          // equal segments; a row of a quad is one 16-byte load and consecutive quads are
+          // consecutive addresses (coalesced). The pending update is added as each element is
+          // read and written back by its only owner. HQ rows per lane are loaded together
+          // (fewer at large blocks, whose register budget is smaller).
+          constexpr int HQ = NT >= 640 ? 4 : 8;
+          const int nq = (n + 3) >> 2;
+          int lq = 0;
+          while (lq < 3 && (nq << (lq + 1)) <= NT) lq++;
+          const int lpr = 1 << lq, seg = (n + lpr - 1) >> lq;
+          for (int r = tid; r < (nq << lq); r += NT) {
+            const int tq = r >> lq, q = r & (lpr - 1);
             const int j0 = q * seg, j1 = min(n, j0 + seg);
-            const float2 ab = AB[t];
-            const float ux = pend ? ab.x : 0.f, uy = pend ? ab.y : 0.f;
+            float ux[4], uy[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              const int t = 4 * tq + u;
+              const float2 ab = (pend && t < n) ? AB[t] : make_float2(0.f, 0.f);
+              ux[u] = ab.x;
+              uy[u] = ab.y;
+            }
             float a1[4] = {0.f, 0.f, 0.f, 0.f}, a2[4] = {0.f, 0.f, 0.f, 0.f};
-            float* col = H + t;
+            float4* base = (float4*)H + tq;  // row j of the quad at base[j * nq]
             int j = j0;
-            for (; j + 7 < j1; j += 8) {
-              float* p0 = col + (size_t)j * n;
-              float h[8];
+            for (; j + HQ - 1 < j1; j += HQ) {
+              float4 h[HQ];
 #pragma unroll
-              for (int u = 0; u < 8; u++) h[u] = p0[(size_t)u * n];
+              for (int u = 0; u < HQ; u++) h[u] = base[(size_t)(j + u) * nq];
 #pragma unroll
-              for (int u = 0; u < 8; u++) {
+              for (int u = 0; u < HQ; u++) {
                 const float2 v = VG[j + u], w = SH[j + u];
-                h[u] = fmaf(uy, w.y, fmaf(ux, w.x, h[u]));
-                a1[u & 3] = fmaf(h[u], v.x, a1[u & 3]);
-                a2[u & 3] = fmaf(h[u], v.y, a2[u & 3]);
+                hquad_step(h[u], v, w, ux, uy, a1, a2);
               }
               if (pend) {
 #pragma unroll
-                for (int u = 0; u < 8; u++) p0[(size_t)u * n] = h[u];
+                for (int u = 0; u < HQ; u++) base[(size_t)(j + u) * nq] = h[u];
               }
             }
             for (; j < j1; j++) {
-              const float2 vj = VG[j], uj = SH[j];
-              const float hv = fmaf(uy, uj.y, fmaf(ux, uj.x, col[(size_t)j * n]));
-              if (pend) col[(size_t)j * n] = hv;
-              a1[0] = fmaf(hv, vj.x, a1[0]);
-              a2[0] = fmaf(hv, vj.y, a2[0]);
+              float4 hv = base[(size_t)j * nq];
+              hquad_step(hv, VG[j], SH[j], ux, uy, a1, a2);
+              if (pend) base[(size_t)j * nq] = hv;
             }
-            float s1 = (a1[0] + a1[1]) + (a1[2] + a1[3]), s2 = (a2[0] + a2[1]) + (a2[2] + a2[3]);
-            if (lsh >= 1) {
-              s1 += dpp_f<DPP_QUAD_XOR1>(s1);
-              s2 += dpp_f<DPP_QUAD_XOR1>(s2);
-            }
-            if (lsh >= 2) {
-              s1 += dpp_f<DPP_QUAD_XOR2>(s1);
-              s2 += dpp_f<DPP_QUAD_XOR2>(s2);
-            }
-            if (lsh >= 3) {
-              s1 += dpp_f<DPP_ROW_HALF_MIRROR>(s1);
-              s2 += dpp_f<DPP_ROW_HALF_MIRROR>(s2);
-            }
-            if (q == 0) {
-              hd[t] = s1;
-              d[t] = s2;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+              float s1 = a1[u], s2 = a2[u];
+              if (lq >= 1) {
+                s1 += dpp_f<DPP_QUAD_XOR1>(s1);
+                s2 += dpp_f<DPP_QUAD_XOR1>(s2);
+              }
+              if (lq >= 2) {
+                s1 += dpp_f<DPP_QUAD_XOR2>(s1);
+                s2 += dpp_f<DPP_QUAD_XOR2>(s2);
+              }
+              if (lq >= 3) {
+                s1 += dpp_f<DPP_ROW_HALF_MIRROR>(s1);
+                s2 += dpp_f<DPP_ROW_HALF_MIRROR>(s2);
+              }
+              const int t = 4 * tq + u;
+              if (q == 0 && t < n) {
+                hd[t] = s1;
+                d[t] = s2;
+              }
             }
           }
           __syncthreads();
@@ -1541,6 +1571,7 @@ const void* nk_kernel_ptr() {
   X(0, 0, false, 64) X(0, 0, false, 128) X(0, 0, false, 192) X(0, 0, false, 256)                         \
   X(0, 0, false, 320) X(0, 0, false, 384) X(0, 0, false, 512) X(0, 0, false, 640)                        \
   X(0, 0, false, 768) X(0, 0, false, 1024)                                                               \
+  X(0, 1, false, 64) X(0, 1, false, 128) X(0, 1, false, 192)                                            \
   X(0, 1, false, 256) X(0, 1, false, 320) X(0, 1, false, 512) X(0, 1, false, 640)                        \
   X(0, 1, false, 768) X(0, 1, false, 1024)                                                               \
   X(0, 2, false, 256) X(0, 2, false, 512) X(0, 2, false, 640) X(0, 2, false, 1024)                       \
@@ -1693,6 +1724,7 @@ int mmffhip_minimize_nk(const mmffhip_batch* b, const mmffhip_nk_opts* o, const 
   rc = make_nk_plan(b, o, pl);
   if (rc) return rc;
   if (workspace_bytes < nk_ws_bytes(pl)) return fail(MMFFHIP_ERR_SIZE, "workspace too small");
+  if ((uintptr_t)workspace & 255) return fail(MMFFHIP_ERR_ARG, "workspace must be 256-byte aligned");
   hipStream_t s = (hipStream_t)stream;
   int32_t* queue = (int32_t*)workspace;
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, queue, 1, 0);

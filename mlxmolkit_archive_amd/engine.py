@@ -73,11 +73,15 @@ def table_lds_bytes(hb: HostBatch) -> np.ndarray:
 
 
 # Size buckets for the nk minimizer: (max 3N, threads per workgroup, Hessian placement).
-# Each bucket is its own persistent launch whose LDS footprint (and hence occupancy) is
-# sized for its largest molecule. 264 = largest n whose packed FP32 inverse Hessian plus
-# vectors fits the 160 KB of LDS a gfx950 workgroup can own.
-DEFAULT_BUCKETS = ((96, 256, _lib.HESS_AUTO), (192, 384, _lib.HESS_AUTO), (264, 640, _lib.HESS_LDS),
-                   (1 << 20, 512, _lib.HESS_GLOBAL))
+# Each bucket is its own persistent launch sized for its largest molecule. Small molecules
+# keep the packed inverse Hessian in LDS. Above n = 96 the full-storage Hessian in global
+# memory (L2/MALL resident, read as coalesced 16-byte column quads) wins: its workgroups need
+# little LDS, so two 256-thread workgroups share a CU (the register file is the limit) and
+# one's memory waits overlap the other's term evaluation. The LDS-resident triangle of a
+# molecule above n = 96 holds a whole CU's LDS for one workgroup. Measured on MI355X on the
+# medium workload (1,000 x 10): 222 ms per step for these buckets, 305 ms for size-matched
+# LDS buckets up to n = 264.
+DEFAULT_BUCKETS = ((96, 128, _lib.HESS_LDS), (1 << 20, 256, _lib.HESS_GLOBAL))
 LBFGS_BUCKETS = ((192, 128, _lib.HESS_LDS), (600, 256, _lib.HESS_LDS), (1 << 20, 512, _lib.HESS_AUTO))
 
 
@@ -186,18 +190,23 @@ class NkRunner:
         return g
 
 
-# candidate (block, placement) per size class for the autotuner
-TUNE_CLASSES = ((96, (128, 256, 384)), (192, (192, 384, 512, 768)), (264, (320, 512, 640, 768)),
-                (1 << 20, (256, 512, 1024)))
+# candidate (block, placement) pairs per size class for the autotuner; LDS candidates
+# whose plan does not fit are skipped
+TUNE_CLASSES = (
+    (96, ((128, _lib.HESS_LDS), (256, _lib.HESS_LDS), (128, _lib.HESS_GLOBAL), (256, _lib.HESS_GLOBAL))),
+    (192, ((192, _lib.HESS_LDS), (384, _lib.HESS_LDS), (128, _lib.HESS_GLOBAL), (192, _lib.HESS_GLOBAL), (256, _lib.HESS_GLOBAL))),
+    (264, ((640, _lib.HESS_LDS), (192, _lib.HESS_GLOBAL), (256, _lib.HESS_GLOBAL), (320, _lib.HESS_GLOBAL))),
+    (1 << 20, ((192, _lib.HESS_GLOBAL), (256, _lib.HESS_GLOBAL), (512, _lib.HESS_GLOBAL))),
+)
 
 
 def autotune_buckets(db: DeviceBatch, x, classes=TUNE_CLASSES, sample: int = 512, reps: int = 1, seed: int = 0,
                      **opts):
-    """Pick the block size (and hence waves per workgroup and workgroups per CU) of each size
-    class by timing candidate launches on a seeded sample of that class's conformers.
+    """Pick the block size (and hence waves per workgroup and workgroups per CU) and the
+    Hessian placement of each size class by timing candidate launches on a seeded sample of
+    that class's conformers.
 
-    Placement follows the class: LDS for classes whose packed Hessian fits, HBM above.
-    Returns a bucket tuple for NkRunner and the timings {bound: {block: ms}}.
+    Returns a bucket tuple for NkRunner and the timings {bound: {(block, placement): ms}}.
     """
     torch = _lib.require_gpu()
     hb = db.host
@@ -205,15 +214,14 @@ def autotune_buckets(db: DeviceBatch, x, classes=TUNE_CLASSES, sample: int = 512
     rng = np.random.default_rng(seed)
     lo = 0
     buckets, timings = [], {}
-    for bound, blocks in classes:
+    for bound, cands in classes:
         idx = np.nonzero((n3 > lo) & (n3 <= bound))[0]
-        place = _lib.HESS_LDS if bound <= 264 else _lib.HESS_GLOBAL
         lo = bound
         if len(idx) == 0:
             continue
         pick = np.sort(rng.choice(idx, size=min(sample, len(idx)), replace=False)).astype(np.int32)
         best, tb = None, {}
-        for blk in blocks:
+        for blk, place in cands:
             try:
                 r = NkRunner(db, order=pick[np.argsort(-n3[pick], kind="stable")], buckets=((1 << 20, blk, place),),
                              **opts)
@@ -227,12 +235,12 @@ def autotune_buckets(db: DeviceBatch, x, classes=TUNE_CLASSES, sample: int = 512
                 r.run(x)
             e1.record()
             torch.cuda.synchronize()
-            tb[blk] = e0.elapsed_time(e1) / reps
-            if best is None or tb[blk] < tb[best]:
-                best = blk
+            tb[(blk, place)] = e0.elapsed_time(e1) / reps
+            if best is None or tb[(blk, place)] < tb[best]:
+                best = (blk, place)
         timings[bound] = tb
         if best is not None:
-            buckets.append((bound, best, place))
+            buckets.append((bound, *best))
     return tuple(buckets), timings
 
 
