@@ -220,14 +220,18 @@ def main():
                      "frac": round(achieved / HBM_PEAK, 4),
                      "traffic": None if traffic is None else round(traffic),
                      "traffic_frac": None if traffic is None else round(traffic / (kern_ms * 1e-3) / HBM_PEAK, 4),
-                     "note": "achieved = algorithmic bytes (12n^2+40n per BFGS iteration + table/vector bytes "
-                             "per evaluation) of one step / the step's GPU time (HIP events around its "
-                             "k_minimize_nk launches); it exceeds the HBM peak because each conformer's "
-                             "Hessian stays on chip: in LDS up to n = 96, above that in L2 and the 256 MB "
-                             "Infinity Cache (one slot per resident workgroup). traffic = measured bytes per "
-                             "step between L2 and memory (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, copy-"
-                             "calibrated; Infinity Cache hits count as traffic); traffic_frac = traffic / "
-                             "step time / HBM peak"},
+                     "note": (("achieved = algorithmic bytes (16mn+48n per L-BFGS iteration, m = 8, + "
+                               "table/vector bytes per evaluation) of one step / the step's GPU time (HIP events "
+                               "around its k_minimize_nk launches); the (s, y) history lives in LDS up to n = 192, "
+                               "in global memory (L2 / Infinity Cache resident) above") if args.lbfgs else
+                              ("achieved = algorithmic bytes (12n^2+40n per BFGS iteration + table/vector bytes "
+                               "per evaluation) of one step / the step's GPU time (HIP events around its "
+                               "k_minimize_nk launches); it exceeds the HBM peak because each conformer's "
+                               "Hessian stays on chip: in LDS up to n = 96, above that in L2 and the 256 MB "
+                               "Infinity Cache (one slot per resident workgroup). traffic = measured bytes per "
+                               "step between L2 and memory (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, copy-"
+                               "calibrated; Infinity Cache hits count as traffic); traffic_frac = traffic / "
+                               "step time / HBM peak"))},
         "cpu_baseline": cpu,
         "precond": pc_info,
         "check": sample_info,

@@ -1577,7 +1577,7 @@ const void* nk_kernel_ptr() {
   X(0, 2, false, 256) X(0, 2, false, 512) X(0, 2, false, 640) X(0, 2, false, 1024)                       \
   X(1, 0, true, 64) X(1, 0, true, 128) X(1, 0, true, 256)                                                \
   X(1, 0, false, 128) X(1, 0, false, 256) X(1, 0, false, 512)                                            \
-  X(1, 1, false, 256) X(1, 1, false, 512)
+  X(1, 1, false, 128) X(1, 1, false, 192) X(1, 1, false, 256) X(1, 1, false, 512)
 
 const void* nk_kernel(int algo, int hm, bool tlds, int nt) {
 #define NKSEL(A, H, TL, T) \

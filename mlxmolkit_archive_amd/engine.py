@@ -82,7 +82,10 @@ def table_lds_bytes(hb: HostBatch) -> np.ndarray:
 # medium workload (1,000 x 10): 222 ms per step for these buckets, 305 ms for size-matched
 # LDS buckets up to n = 264.
 DEFAULT_BUCKETS = ((96, 128, _lib.HESS_LDS), (1 << 20, 256, _lib.HESS_GLOBAL))
-LBFGS_BUCKETS = ((192, 128, _lib.HESS_LDS), (600, 256, _lib.HESS_LDS), (1 << 20, 512, _lib.HESS_AUTO))
+# L-BFGS keeps its (s, y) history in LDS for small molecules and in global memory above
+# n = 192 (256 threads): chain workload subset (250 x 8, n = 450-1200), 855 ms per step
+# against 1071 ms with LDS history up to n = 600.
+LBFGS_BUCKETS = ((192, 128, _lib.HESS_LDS), (1 << 20, 256, _lib.HESS_GLOBAL))
 
 
 class NkRunner:
