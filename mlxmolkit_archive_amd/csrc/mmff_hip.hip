@@ -467,14 +467,17 @@ __device__ __forceinline__ void hrow_load4(const float* H, const float2* VG, int
   w[3] = VG[j + 3];
 }
 
-// one row of a column quad of the full Hessian: add the pending update (ux, uy per column,
-// w = {s, h} of the row) and accumulate the row's {dg, g} = v into the quad's two dot products
-__device__ __forceinline__ void hquad_step(float4& h, float2 v, float2 w, const float (&ux)[4],
-                                           const float (&uy)[4], float (&a1)[4], float (&a2)[4]) {
+// one row of a column quad of the full Hessian gains a pending rank-2 update: column t's
+// element gains ux[t] s + uy[t] h, with w = {s, h} of the row
+__device__ __forceinline__ void hquad_upd(float4& h, float2 w, const float (&ux)[4], const float (&uy)[4]) {
   h.x = fmaf(uy[0], w.y, fmaf(ux[0], w.x, h.x));
   h.y = fmaf(uy[1], w.y, fmaf(ux[1], w.x, h.y));
   h.z = fmaf(uy[2], w.y, fmaf(ux[2], w.x, h.z));
   h.w = fmaf(uy[3], w.y, fmaf(ux[3], w.x, h.w));
+}
+
+// accumulate the row's {dg, g} = v into the quad's two dot products
+__device__ __forceinline__ void hquad_dot(const float4& h, float2 v, float (&a1)[4], float (&a2)[4]) {
   a1[0] = fmaf(h.x, v.x, a1[0]);
   a2[0] = fmaf(h.x, v.y, a2[0]);
   a1[1] = fmaf(h.y, v.x, a1[1]);
@@ -483,6 +486,38 @@ __device__ __forceinline__ void hquad_step(float4& h, float2 v, float2 w, const 
   a2[2] = fmaf(h.z, v.y, a2[2]);
   a1[3] = fmaf(h.w, v.x, a1[3]);
   a2[3] = fmaf(h.w, v.y, a2[3]);
+}
+
+// Rows [j, j1) of a column quad of the full Hessian (row r at base[r * nq]): apply the NU
+// oldest of the pending rank-2 updates in order (update k: row {s, h} in S[k], column
+// coefficients cx[k], cy[k]), accumulate H v for v = {dg, g}, and write the rows back when
+// all NPM slots were applied. HQ rows are loaded together.
+template <int HQ, int NU, int NPM>
+__device__ __forceinline__ void hquad_rows(float4* base, int nq, int j, int j1, const float2* VG,
+                                           const float2* const (&S)[NPM], const float (&cx)[NPM][4],
+                                           const float (&cy)[NPM][4], float (&a1)[4], float (&a2)[4]) {
+  for (; j + HQ - 1 < j1; j += HQ) {
+    float4 h[HQ];
+#pragma unroll
+    for (int u = 0; u < HQ; u++) h[u] = base[(size_t)(j + u) * nq];
+#pragma unroll
+    for (int u = 0; u < HQ; u++) {
+#pragma unroll
+      for (int k = 0; k < NU; k++) hquad_upd(h[u], S[k][j + u], cx[k], cy[k]);
+      hquad_dot(h[u], VG[j + u], a1, a2);
+    }
+    if (NU == NPM) {
+#pragma unroll
+      for (int u = 0; u < HQ; u++) base[(size_t)(j + u) * nq] = h[u];
+    }
+  }
+  for (; j < j1; j++) {
+    float4 h = base[(size_t)j * nq];
+#pragma unroll
+    for (int k = 0; k < NU; k++) hquad_upd(h, S[k][j], cx[k], cy[k]);
+    hquad_dot(h, VG[j], a1, a2);
+    if (NU == NPM) base[(size_t)j * nq] = h;
+  }
 }
 
 // Row groups [ka, kb) (rows 4k..4k+3) of one column of the padded packed matrix gain
@@ -599,6 +634,8 @@ constexpr int MAX_LS_ITERS = 1000;
 constexpr float GRAD_SCALE_INIT = 0.1f;
 constexpr double GRAD_CAP = 10.0;
 constexpr int LBFGS_MAX_M = 16;
+// pending rank-2 updates the global-Hessian BFGS pass applies on the fly before writing H back
+constexpr int HPEND_MAX = 2;
 
 struct NkParams {
   const float* x_in;
@@ -638,6 +675,7 @@ __host__ __device__ inline size_t nk_lds_bytes(int algo, bool hlds, int nt, int 
   b += (size_t)max_n * 4 * 4;   // xs, d, xo, hd
   b += (size_t)max_n * 16;      // V = {dg, g, pending xi, pending hd}
   b += (size_t)max_n * 8;       // per-lane coefficients of the pending update
+  if (algo == MMFFHIP_BFGS && !hlds) b += (size_t)(HPEND_MAX - 1) * max_n * 16;  // more pending updates
   if (algo == MMFFHIP_LBFGS) {
     if (hlds) b += 2 * (size_t)m * max_n * 4;
   } else if (hlds) {
@@ -675,7 +713,14 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
   float* xo = d + mn;   // previous x, then xi = x - x_old
   float* hd = xo + mn;  // H dg (BFGS) / two-loop q (L-BFGS)
   float2* AB = (float2*)(hd + mn);  // pending update U(t, j) = AB[t].x xi_j + AB[t].y hd_j
-  float* big = (float*)(AB + mn);
+  // global-Hessian BFGS keeps up to HPEND_MAX pending updates (slot 0 in SH/AB, slot k >= 1
+  // in XS) and writes H back once every HPEND_MAX iterations
+  constexpr bool LAZY = HM == 1 && ALGO == MMFFHIP_BFGS;
+  constexpr int NPM = LAZY ? HPEND_MAX : 1;
+  float2* XS = AB + mn;
+  auto sh_slot = [&](int k) { return k == 0 ? SH : XS + (size_t)(k - 1) * 2 * mn; };
+  auto ab_slot = [&](int k) { return k == 0 ? AB : XS + (size_t)(k - 1) * 2 * mn + mn; };
+  float* big = (float*)(XS + (size_t)(NPM - 1) * 2 * mn);
   float* H = HLDS ? big : P.hglob + (size_t)blockIdx.x * P.slot_floats;
   size_t big_floats = 0;
   if (HLDS) big_floats = ALGO == MMFFHIP_BFGS ? packed_floats(mn) : 2 * (size_t)P.m * mn;
@@ -780,7 +825,7 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
     int status = 1;
     int hist_count = 0, hist_idx = 0;
     bool fallback_used = (pc == nullptr);
-    bool pend = false;  // a rank-2 update is pending (applied lazily in the next H pass)
+    int npend = 0;  // rank-2 updates pending (applied lazily in later H passes)
 
     for (int it = 0; it < P.max_iters; it++) {
       iters = it + 1;
@@ -866,7 +911,7 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
           // learned preconditioner fallback: restart as identity-scaled BFGS
           fallback_used = true;
           pc = nullptr;
-          pend = false;
+          npend = 0;
           init_h();
           for (int i = tid; i < n; i += NT) d[i] = -VG[i].y;
           __syncthreads();
@@ -993,9 +1038,12 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
           // full storage in HBM with rows padded to ldh = round4(n) floats (16-byte aligned).
           // 2^lq consecutive lanes share the column quad 4tq..4tq+3 and split its rows into
           // equal segments; a row of a quad is one 16-byte load and consecutive quads are
-          // consecutive addresses (coalesced). The pending update is added as each element is
-          // read and written back by its only owner. HQ rows per lane are loaded together
-          // (fewer at large blocks, whose register budget is smaller).
+          // consecutive addresses (coalesced). Pending updates are added, oldest first, as each
+          // element is read; when all NPM slots are pending the rows are written back by their
+          // only owner, so H is written once every NPM iterations. Adding the updates one after
+          // the other rounds each element exactly as writing back after every update would.
+          // HQ rows per lane are loaded together (fewer at large blocks, whose register budget
+          // is smaller).
           constexpr int HQ = NT >= 640 ? 4 : 8;
           const int nq = (n + 3) >> 2;
           int lq = 0;
@@ -1004,36 +1052,33 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
           for (int r = tid; r < (nq << lq); r += NT) {
             const int tq = r >> lq, q = r & (lpr - 1);
             const int j0 = q * seg, j1 = min(n, j0 + seg);
-            float ux[4], uy[4];
+            float cx[NPM][4], cy[NPM][4];
+            const float2* S[NPM];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-              const int t = 4 * tq + u;
-              const float2 ab = (pend && t < n) ? AB[t] : make_float2(0.f, 0.f);
-              ux[u] = ab.x;
-              uy[u] = ab.y;
+            for (int k = 0; k < NPM; k++) {
+              S[k] = sh_slot(k);
+              const float2* ab = ab_slot(k);
+#pragma unroll
+              for (int u = 0; u < 4; u++) {
+                const int t = 4 * tq + u;
+                const float2 c = (npend > k && t < n) ? ab[t] : make_float2(0.f, 0.f);
+                cx[k][u] = c.x;
+                cy[k][u] = c.y;
+              }
             }
             float a1[4] = {0.f, 0.f, 0.f, 0.f}, a2[4] = {0.f, 0.f, 0.f, 0.f};
             float4* base = (float4*)H + tq;  // row j of the quad at base[j * nq]
-            int j = j0;
-            for (; j + HQ - 1 < j1; j += HQ) {
-              float4 h[HQ];
-#pragma unroll
-              for (int u = 0; u < HQ; u++) h[u] = base[(size_t)(j + u) * nq];
-#pragma unroll
-              for (int u = 0; u < HQ; u++) {
-                const float2 v = VG[j + u], w = SH[j + u];
-                hquad_step(h[u], v, w, ux, uy, a1, a2);
-              }
-              if (pend) {
-#pragma unroll
-                for (int u = 0; u < HQ; u++) base[(size_t)(j + u) * nq] = h[u];
-              }
-            }
-            for (; j < j1; j++) {
-              float4 hv = base[(size_t)j * nq];
-              hquad_step(hv, VG[j], SH[j], ux, uy, a1, a2);
-              if (pend) base[(size_t)j * nq] = hv;
-            }
+            static_assert(NPM <= 4, "at most four pending updates");
+            if (npend == 4 && NPM >= 4)
+              hquad_rows<HQ, NPM >= 4 ? 4 : 0, NPM>(base, nq, j0, j1, VG, S, cx, cy, a1, a2);
+            else if (npend == 3 && NPM >= 3)
+              hquad_rows<HQ, NPM >= 3 ? 3 : 0, NPM>(base, nq, j0, j1, VG, S, cx, cy, a1, a2);
+            else if (npend == 2 && NPM >= 2)
+              hquad_rows<HQ, NPM >= 2 ? 2 : 0, NPM>(base, nq, j0, j1, VG, S, cx, cy, a1, a2);
+            else if (npend == 1)
+              hquad_rows<HQ, 1, NPM>(base, nq, j0, j1, VG, S, cx, cy, a1, a2);
+            else
+              hquad_rows<HQ, 0, NPM>(base, nq, j0, j1, VG, S, cx, cy, a1, a2);
 #pragma unroll
             for (int u = 0; u < 4; u++) {
               float s1 = a1[u], s2 = a2[u];
@@ -1057,6 +1102,7 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
             }
           }
           __syncthreads();
+          if (npend == NPM) npend = 0;  // every pending update is in H now
         }
         double v6[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
         for (int i = tid; i < n; i += NT) {
@@ -1077,20 +1123,22 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
           const double fi = 1.0 / fac, fei = 1.0 / fae;
           const double A = fi + fae * fi * fi, Bc = -fae * fi * fei, Cc = fae * fei * fei - fei;
           const double xig = v6[4], hdg = v6[5];
+          // the new update goes behind the pending ones (slot 0 when H is kept updated)
+          float2* sh_new = sh_slot(LAZY ? npend : 0);
+          float2* ab_new = ab_slot(LAZY ? npend : 0);
           for (int t = tid; t < n; t += NT) {
             const double xi = xo[t], h = hd[t];
             const double al = A * xi + Bc * h, be = Bc * xi + Cc * h;
             d[t] = -(float)((double)d[t] + al * xig + be * hdg);
-            SH[t] = make_float2((float)xi, (float)h);
-            AB[t] = make_float2((float)al, (float)be);
+            sh_new[t] = make_float2((float)xi, (float)h);
+            ab_new[t] = make_float2((float)al, (float)be);
           }
-          pend = true;
+          npend++;
         } else {
           for (int t = tid; t < n; t += NT) d[t] = -d[t];
-          pend = false;
         }
         __syncthreads();
-        if (PACKED && pend) {
+        if (PACKED && npend) {
           // write the update into the packed triangle: column c gains alpha_i s_c + beta_i h_c
           // in rows i <= c. Columns c and n-1-c form a pair and 2^lsh2 consecutive lanes split
           // the pair's 4-row groups evenly; each lane walks its groups in float4 steps with
@@ -1105,7 +1153,7 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
             if (f0 < l1) hcol_update4(H + pcol(c1), AB, SH[c1], f0, min(f1, l1));
             if (two && f1 > l1) hcol_update4(H + pcol(c2), AB, SH[c2], max(f0, l1) - l1, f1 - l1);
           }
-          pend = false;
+          npend = 0;
           __syncthreads();
         }
       } else {
