@@ -100,8 +100,11 @@ typedef struct {
                             restarts from the identity if a line search fails. NULL = identity
                             (reference) */
   int32_t table_lds_bytes; /* >0: stage each molecule's term tables in LDS, reserving this many
-                              bytes (ignored when they do not fit beside the Hessian); a
-                              conformer whose tables need more gets status 2 and E = NaN */
+                              bytes: all tables beside an LDS Hessian; beside a global Hessian
+                              only the bonded tables (pairs stay in global memory), and only
+                              when that keeps the workgroups per CU. Ignored when they do not
+                              fit. A conformer whose staged tables need more gets status 2 and
+                              E = NaN */
 } mmffhip_nk_opts;
 
 int mmffhip_abi_version(void);

@@ -147,9 +147,9 @@ struct LdsAcc {
       atomicAdd(&g[3 * a], fx);
       atomicAdd(&g[3 * a + 1], fy);
       atomicAdd(&g[3 * a + 2], fz);
-      atomicAdd(&g[3 * b], 0ull - fx);
-      atomicAdd(&g[3 * b + 1], 0ull - fy);
-      atomicAdd(&g[3 * b + 2], 0ull - fz);
+      atomicSub(&g[3 * b], fx);
+      atomicSub(&g[3 * b + 1], fy);
+      atomicSub(&g[3 * b + 2], fz);
     }
   }
 };
@@ -217,9 +217,12 @@ __device__ __forceinline__ void stage_copy(T* dst, const T* src, int count, int 
 }
 
 // Copy a molecule's tables into LDS at `base` (16-byte aligned) and return a view of them.
+// Without `pairs` only the bonded tables are copied and the view reads pairs from global
+// memory (their loop streams them with loads one step ahead).
 __device__ __forceinline__ TView stage_tables(const KBatch& B, const MolDesc& m, unsigned char* base, int tid,
-                                              int nt) {
+                                              int nt, bool pairs) {
   TView v;
+  const int npair = pairs ? m.npair : 0;
   unsigned char* p = base;
   auto take = [&](size_t bytes) {
     unsigned char* q = p;
@@ -228,7 +231,7 @@ __device__ __forceinline__ TView stage_tables(const KBatch& B, const MolDesc& m,
   };
   float4* tor_p = (float4*)take(16 * (size_t)m.ntor);
   float4* sb_p = (float4*)take(16 * (size_t)m.nsb);
-  float4* pair_p = (float4*)take(16 * (size_t)m.npair);
+  float4* pair_p = (float4*)take(16 * (size_t)npair);
   uint2* ang_idx = (uint2*)take(8 * (size_t)m.na);
   float2* ang_p = (float2*)take(8 * (size_t)m.na);
   uint2* sb_idx = (uint2*)take(8 * (size_t)m.nsb);
@@ -238,10 +241,10 @@ __device__ __forceinline__ TView stage_tables(const KBatch& B, const MolDesc& m,
   uint32_t* bond_ij = (uint32_t*)take(4 * (size_t)m.nb);
   float* sb_t0 = (float*)take(4 * (size_t)m.nsb);
   float* oop_k = (float*)take(4 * (size_t)m.noop);
-  uint32_t* pair_ij = (uint32_t*)take(4 * (size_t)m.npair);
+  uint32_t* pair_ij = (uint32_t*)take(4 * (size_t)npair);
   stage_copy(tor_p, B.tor_p + m.tor_off, m.ntor, tid, nt);
   stage_copy(sb_p, B.sb_p + m.sb_off, m.nsb, tid, nt);
-  stage_copy(pair_p, B.pair_p + m.pair_off, m.npair, tid, nt);
+  stage_copy(pair_p, B.pair_p + m.pair_off, npair, tid, nt);
   stage_copy(ang_idx, B.ang_idx + m.ang_off, m.na, tid, nt);
   stage_copy(ang_p, B.ang_p + m.ang_off, m.na, tid, nt);
   stage_copy(sb_idx, B.sb_idx + m.sb_off, m.nsb, tid, nt);
@@ -251,7 +254,7 @@ __device__ __forceinline__ TView stage_tables(const KBatch& B, const MolDesc& m,
   stage_copy(bond_ij, B.bond_ij + m.bond_off, m.nb, tid, nt);
   stage_copy(sb_t0, B.sb_t0 + m.sb_off, m.nsb, tid, nt);
   stage_copy(oop_k, B.oop_k + m.oop_off, m.noop, tid, nt);
-  stage_copy(pair_ij, B.pair_ij + m.pair_off, m.npair, tid, nt);
+  stage_copy(pair_ij, B.pair_ij + m.pair_off, npair, tid, nt);
   v.bond_ij = bond_ij;
   v.bond_p = bond_p;
   v.ang_idx = ang_idx;
@@ -263,8 +266,8 @@ __device__ __forceinline__ TView stage_tables(const KBatch& B, const MolDesc& m,
   v.oop_k = oop_k;
   v.tor_idx = tor_idx;
   v.tor_p = tor_p;
-  v.pair_ij = pair_ij;
-  v.pair_p = pair_p;
+  v.pair_ij = pairs ? pair_ij : B.pair_ij + m.pair_off;
+  v.pair_p = pairs ? (const float4*)pair_p : B.pair_p + m.pair_off;
   return v;
 }
 
@@ -361,35 +364,21 @@ __device__ __forceinline__ double eval_terms(const TView& TV, const MolDesc& m, 
     cur = nxt;
     t = tn;
   }
-  // non-bonded pairs: two pairs per lane per step (independent dependency chains),
-  // the next two prefetched
-  if (mask & 96u) {
+  // non-bonded pairs: two pairs per lane per step (independent dependency chains), the
+  // next two loaded while the current two are computed. The loads are unconditional (index
+  // clamped to the last pair; such duplicates are never used), so the wait before the
+  // current pairs' use leaves the next pairs' loads in flight.
+  if ((mask & 96u) && m.npair > 0) {
     const float ke = (mask & 32u) ? 1.0f : 0.0f, kq = (mask & 64u) ? 1.0f : 0.0f;
-    const int np = m.npair;
+    const int np = m.npair, last = np - 1;
     int u = threadIdx.x;
-    uint32_t ija = 0, ijb = 0;
-    float4 pa = make_float4(1.f, 0.f, 0.f, 0.f), pb = pa;
-    if (u < np) {
-      ija = TV.pair_ij[u];
-      pa = TV.pair_p[u];
-    }
-    if (u + NT < np) {
-      ijb = TV.pair_ij[u + NT];
-      pb = TV.pair_p[u + NT];
-    }
+    uint32_t ija = TV.pair_ij[min(u, last)], ijb = TV.pair_ij[min(u + NT, last)];
+    float4 pa = TV.pair_p[min(u, last)], pb = TV.pair_p[min(u + NT, last)];
     double ev_acc = 0.0, ee_acc = 0.0;
     while (u < np) {
       const int un = u + 2 * NT;
-      uint32_t nija = 0, nijb = 0;
-      float4 npa = make_float4(1.f, 0.f, 0.f, 0.f), npb = npa;
-      if (un < np) {
-        nija = TV.pair_ij[un];
-        npa = TV.pair_p[un];
-      }
-      if (un + NT < np) {
-        nijb = TV.pair_ij[un + NT];
-        npb = TV.pair_p[un + NT];
-      }
+      const uint32_t nija = TV.pair_ij[min(un, last)], nijb = TV.pair_ij[min(un + NT, last)];
+      const float4 npa = TV.pair_p[min(un, last)], npb = TV.pair_p[min(un + NT, last)];
       float eva, eea;
       t_pair<FAM, G>(xs, acc, (int)(ija & 0xffff), (int)(ija >> 16), pa.x, ke * pa.y, kq * pa.z, diel, eva, eea);
       ev_acc += eva;
@@ -654,7 +643,7 @@ struct NkParams {
   int m;
   int max_n;          // 3 * max_atoms rounded up to a multiple of 4
   int ls_grad;
-  int tbytes;         // LDS bytes reserved for staged term tables (TLDS kernels)
+  int tbytes;         // LDS bytes reserved for staged term tables (TL kernels)
 };
 
 // Packed upper-triangle storage of a symmetric n x n matrix with every column padded to a
@@ -694,7 +683,9 @@ __device__ __forceinline__ float h_init(const float* pc, float psc, int i, int j
 }
 
 // HM: where the dense inverse Hessian lives -- 0 packed in LDS, 1 full in HBM, 2 packed in HBM
-template <int ALGO, int HM, bool TLDS, int NT>
+// TL: term tables read from global memory (0), staged in LDS (1), or bonded tables staged and
+// pairs read from global memory (2)
+template <int ALGO, int HM, int TL, int NT>
 __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
   constexpr bool HLDS = HM == 0;    // LDS-resident (also selects the LDS L-BFGS history)
   constexpr bool PACKED = HM != 1;  // packed upper triangle, select read + column-pair write-back
@@ -751,8 +742,8 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
     while (lsh2 < 3 && (np << (lsh2 + 1)) <= NT) lsh2++;
     const float* pc = P.precond ? P.precond + (size_t)(off / 3) * 9 : nullptr;
     TView TV;
-    if (TLDS) {
-      if (table_lds_bytes(m.nb, m.na, m.nsb, m.noop, m.ntor, m.npair) > (size_t)P.tbytes) {
+    if (TL) {
+      if (table_lds_bytes(m.nb, m.na, m.nsb, m.noop, m.ntor, TL == 1 ? m.npair : 0) > (size_t)P.tbytes) {
         // tables larger than the reserved LDS: report and skip (the planner never does this)
         for (int i = tid; i < n; i += NT) P.x_out[off + i] = P.x_in[off + i];
         if (tid == 0) {
@@ -762,7 +753,7 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
         }
         continue;
       }
-      TV = stage_tables(B, m, tstage, tid, NT);
+      TV = stage_tables(B, m, tstage, tid, NT, TL == 1);
     } else {
       TV = global_view(B, m);
     }
@@ -1600,7 +1591,7 @@ int round4(int v) { return (v + 3) & ~3; }
 struct NkPlan {
   int nt = 256;
   int hm = 0;  // Hessian placement of k_minimize_nk (0 packed LDS, 1 full HBM, 2 packed HBM)
-  bool tlds = false;
+  int tl = 0;  // term tables: 0 global, 1 all staged in LDS, 2 bonded staged
   int tbytes = 0;
   size_t lds = 0;
   int grid = 0;
@@ -1608,28 +1599,26 @@ struct NkPlan {
   int max_n = 0;
 };
 
-template <int ALGO, int HM, bool TLDS, int NT>
+template <int ALGO, int HM, int TL, int NT>
 const void* nk_kernel_ptr() {
-  return (const void*)&k_minimize_nk<ALGO, HM, TLDS, NT>;
+  return (const void*)&k_minimize_nk<ALGO, HM, TL, NT>;
 }
 
 #define NK_FOR_EACH(X)                                                                                   \
-  X(0, 0, true, 64) X(0, 0, true, 128) X(0, 0, true, 192) X(0, 0, true, 256)                             \
-  X(0, 0, true, 384) X(0, 0, true, 512)                                                                  \
-  X(0, 0, false, 64) X(0, 0, false, 128) X(0, 0, false, 192) X(0, 0, false, 256)                         \
-  X(0, 0, false, 320) X(0, 0, false, 384) X(0, 0, false, 512) X(0, 0, false, 640)                        \
-  X(0, 0, false, 768) X(0, 0, false, 1024)                                                               \
-  X(0, 1, false, 64) X(0, 1, false, 128) X(0, 1, false, 192)                                            \
-  X(0, 1, false, 256) X(0, 1, false, 320) X(0, 1, false, 512) X(0, 1, false, 640)                        \
-  X(0, 1, false, 768) X(0, 1, false, 1024)                                                               \
-  X(0, 2, false, 256) X(0, 2, false, 512) X(0, 2, false, 640) X(0, 2, false, 1024)                       \
-  X(1, 0, true, 64) X(1, 0, true, 128) X(1, 0, true, 256)                                                \
-  X(1, 0, false, 128) X(1, 0, false, 256) X(1, 0, false, 512)                                            \
-  X(1, 1, false, 128) X(1, 1, false, 192) X(1, 1, false, 256) X(1, 1, false, 512)
+  X(0, 0, 1, 64) X(0, 0, 1, 128) X(0, 0, 1, 192) X(0, 0, 1, 256) X(0, 0, 1, 384) X(0, 0, 1, 512)        \
+  X(0, 0, 0, 64) X(0, 0, 0, 128) X(0, 0, 0, 192) X(0, 0, 0, 256) X(0, 0, 0, 320) X(0, 0, 0, 384)        \
+  X(0, 0, 0, 512) X(0, 0, 0, 640) X(0, 0, 0, 768) X(0, 0, 0, 1024)                                     \
+  X(0, 1, 0, 64) X(0, 1, 0, 128) X(0, 1, 0, 192) X(0, 1, 0, 256) X(0, 1, 0, 320) X(0, 1, 0, 512)        \
+  X(0, 1, 0, 640) X(0, 1, 0, 768) X(0, 1, 0, 1024)                                                     \
+  X(0, 1, 2, 128) X(0, 1, 2, 192) X(0, 1, 2, 256) X(0, 1, 2, 320) X(0, 1, 2, 512)                      \
+  X(0, 2, 0, 256) X(0, 2, 0, 512) X(0, 2, 0, 640) X(0, 2, 0, 1024)                                     \
+  X(1, 0, 1, 64) X(1, 0, 1, 128) X(1, 0, 1, 256)                                                       \
+  X(1, 0, 0, 128) X(1, 0, 0, 256) X(1, 0, 0, 512)                                                      \
+  X(1, 1, 0, 128) X(1, 1, 0, 192) X(1, 1, 0, 256) X(1, 1, 0, 512) X(1, 1, 2, 256)
 
-const void* nk_kernel(int algo, int hm, bool tlds, int nt) {
+const void* nk_kernel(int algo, int hm, int tl, int nt) {
 #define NKSEL(A, H, TL, T) \
-  if (algo == A && hm == H && tlds == TL && nt == T) return nk_kernel_ptr<A, H, TL, T>();
+  if (algo == A && hm == H && tl == TL && nt == T) return nk_kernel_ptr<A, H, TL, T>();
   NK_FOR_EACH(NKSEL)
 #undef NKSEL
   return nullptr;
@@ -1664,16 +1653,29 @@ int make_nk_plan(const mmffhip_batch* b, const mmffhip_nk_opts* o, NkPlan& pl) {
   }
   pl.lds = pl.hm == 0 ? lds_in : lds_out;
   if (pl.lds > (size_t)di.lds) return fail(MMFFHIP_ERR_SIZE, "plan exceeds the LDS of one workgroup");
-  // stage term tables in LDS when asked and when they fit next to the Hessian
+  // Stage term tables in LDS when asked and when they fit: all of them next to an LDS
+  // Hessian; only the bonded ones beside a global Hessian, and only when that keeps the
+  // unstaged kernel's workgroups per CU (there, occupancy hides the pair loads).
   if (o->table_lds_bytes > 0) {
+    const int tl = pl.hm == 0 ? 1 : 2;
     const size_t with = nk_lds_bytes(o->algo, pl.hm == 0, pl.nt, pl.max_n, m, (size_t)o->table_lds_bytes);
-    if (with <= (size_t)di.lds && nk_kernel(o->algo, pl.hm, true, pl.nt)) {
-      pl.tlds = true;
+    const void* kt = nk_kernel(o->algo, pl.hm, tl, pl.nt);
+    bool take = with <= (size_t)di.lds && kt != nullptr;
+    if (take && tl == 2) {
+      const void* k0 = nk_kernel(o->algo, pl.hm, 0, pl.nt);
+      int pc0 = 0, pc1 = 0;
+      allow_lds(kt, with);
+      if (k0) allow_lds(k0, pl.lds);
+      take = k0 && hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc0, k0, pl.nt, pl.lds) == hipSuccess &&
+             hipOccupancyMaxActiveBlocksPerMultiprocessor(&pc1, kt, pl.nt, with) == hipSuccess && pc1 >= pc0;
+    }
+    if (take) {
+      pl.tl = tl;
       pl.tbytes = ((o->table_lds_bytes + 15) / 16) * 16;
       pl.lds = with;
     }
   }
-  const void* kp = nk_kernel(o->algo, pl.hm, pl.tlds, pl.nt);
+  const void* kp = nk_kernel(o->algo, pl.hm, pl.tl, pl.nt);
   if (!kp) return fail(MMFFHIP_ERR_ARG, "no kernel instantiated for this (algo, placement, block) combination");
   allow_lds(kp, pl.lds);
   int per_cu = 0;
@@ -1692,14 +1694,14 @@ int make_nk_plan(const mmffhip_batch* b, const mmffhip_nk_opts* o, NkPlan& pl) {
 
 size_t nk_ws_bytes(const NkPlan& pl) { return 256 + (size_t)pl.grid * pl.slot_floats * 4; }
 
-template <int ALGO, int HM, bool TLDS, int NT>
+template <int ALGO, int HM, int TL, int NT>
 void launch_nk_t(const KBatch& kb, const NkParams& P, const NkPlan& pl, hipStream_t s) {
-  hipLaunchKernelGGL((k_minimize_nk<ALGO, HM, TLDS, NT>), dim3(pl.grid), dim3(NT), pl.lds, s, kb, P);
+  hipLaunchKernelGGL((k_minimize_nk<ALGO, HM, TL, NT>), dim3(pl.grid), dim3(NT), pl.lds, s, kb, P);
 }
 
 void launch_nk(int algo, const KBatch& kb, const NkParams& P, const NkPlan& pl, hipStream_t s) {
 #define NKL(A, H, TL, T)                                          \
-  if (algo == A && pl.hm == H && pl.tlds == TL && pl.nt == T) {   \
+  if (algo == A && pl.hm == H && pl.tl == TL && pl.nt == T) {     \
     launch_nk_t<A, H, TL, T>(kb, P, pl, s);                       \
     return;                                                       \
   }

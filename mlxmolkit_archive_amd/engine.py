@@ -65,10 +65,12 @@ def nk_options(max_iters=500, grad_tol=1e-4, use_lbfgs=False, lbfgs_m=8, diel=1,
     return o
 
 
-def table_lds_bytes(hb: HostBatch) -> np.ndarray:
-    """Per-molecule bytes of the LDS staging layout of its term tables (csrc table_lds_bytes)."""
+def table_lds_bytes(hb: HostBatch, pairs: bool = True) -> np.ndarray:
+    """Per-molecule bytes of the LDS staging layout of its term tables (csrc table_lds_bytes);
+    without ``pairs``, of its bonded tables only (what a global-Hessian launch stages)."""
     d = hb.mol_desc.reshape(-1, 16).astype(np.int64)
     nb, na, nsb, noop, nt, npair = d[:, 2], d[:, 4], d[:, 6], d[:, 8], d[:, 10], d[:, 12]
+    npair = npair if pairs else 0 * npair
     return 16 * (nt + nsb + npair) + 8 * (2 * na + nsb + noop + nt + nb) + 4 * (nb + nsb + noop + npair)
 
 
@@ -106,6 +108,7 @@ class NkRunner:
         user_place = int(opts.pop("hess_place", _lib.HESS_AUTO) or 0)
         stage = bool(opts.pop("stage_tables", True))
         tbytes_mol = table_lds_bytes(db.host)
+        bbytes_mol = table_lds_bytes(db.host, pairs=False)
         if buckets is None:
             buckets = LBFGS_BUCKETS if use_lbfgs else DEFAULT_BUCKETS
         hb = db.host
@@ -125,8 +128,10 @@ class NkRunner:
             desc = _lib.Batch.from_buffer_copy(db.desc)
             desc.n_confs = len(sel)
             desc.max_atoms = int(hb.n_atoms[hb.conf_mol[sel]].max())
-            tb = int(tbytes_mol[hb.conf_mol[sel]].max()) if stage else 0
-            o = nk_options(block=user_block or block, hess_place=user_place or place, table_lds_bytes=tb, **opts)
+            hp = user_place or place
+            per_mol = bbytes_mol if hp == _lib.HESS_GLOBAL else tbytes_mol
+            tb = int(per_mol[hb.conf_mol[sel]].max()) if stage else 0
+            o = nk_options(block=user_block or block, hess_place=hp, table_lds_bytes=tb, **opts)
             nbytes = C.c_size_t()
             _lib.check(L.mmffhip_minimize_nk_workspace(C.byref(desc), C.byref(o), C.byref(nbytes)))
             ws = torch.empty(int(nbytes.value), dtype=torch.uint8, device=dev)
