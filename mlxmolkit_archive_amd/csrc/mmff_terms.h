@@ -51,6 +51,11 @@ __device__ __forceinline__ float clampf(float v, float lo, float hi) { return fm
 
 __device__ __forceinline__ f3 ld(const float* x, int a) { return {x[3 * a], x[3 * a + 1], x[3 * a + 2]}; }
 
+// 1/x with one v_rcp_f32 (within 1 ulp) for x positive and far from the denormal range, as
+// every argument below is (a buffered or floored denominator); the generic division adds
+// a range reduction around the same instruction.
+__device__ __forceinline__ float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // ----------------------------------------------------------------------------------
 // bond stretch: TG mmff_bfgs_header.metal:15-46, MEGA mmff_metal_kernel.py:334-345
 template <int FAM, bool G, class Acc>
@@ -337,16 +342,16 @@ __device__ __forceinline__ void t_pair(const float* x, Acc& acc, int i, int j, f
     const float d2c = fmaxf(d2, 1e-16f);
     const float ir = rsqrtf(d2c);
     const float r = d2c * ir;
-    const float iR = 1.0f / fmaxf(R, 1e-8f);
+    const float iR = rcp(fmaxf(R, 1e-8f));
     const float q = r * iR;
     const float q2 = q * q, q6 = q2 * q2 * q2, q7 = q6 * q;
-    const float rq = 1.0f / (q + 0.07f);
+    const float rq = rcp(q + 0.07f);
     const float t = 1.07f * rq;
     const float t2 = t * t, t7 = t2 * t2 * t2 * t;
-    const float r7 = 1.0f / (q7 + 0.12f);  // q >= 0: q7 + 0.12 >= 0.12
+    const float r7 = rcp(q7 + 0.12f);  // q >= 0: q7 + 0.12 >= 0.12
     ev = eps * t7 * (1.12f * r7 - 2.0f);
     const float dpd = r + 0.05f;
-    const float rd = 1.0f / dpd;  // dpd >= 0.05
+    const float rd = rcp(dpd);  // dpd >= 0.05
     const float qe = 332.0716f * qqs;
     ee = diel == 2 ? qe * rd * rd : qe * rd;
     if (G) {
@@ -359,22 +364,22 @@ __device__ __forceinline__ void t_pair(const float* x, Acc& acc, int i, int j, f
   } else {
     // shared reciprocals: 1/R, 1/(rho + 0.07), 1/(rho^7 + 0.12), 1/(r + 0.05), 1/r
     const float r = sqrtf(d2) + M_KEPS;
-    const float iR = 1.0f / R;
+    const float iR = rcp(R);
     const float rho = r * iR;
     const float rho2 = rho * rho, rho6 = rho2 * rho2 * rho2, rho7 = rho6 * rho;
-    const float rq = 1.0f / (rho + 0.07f);
-    const float r7 = 1.0f / (rho7 + 0.12f);
+    const float rq = rcp(rho + 0.07f);
+    const float r7 = rcp(rho7 + 0.12f);
     const float t1 = 1.07f * rq;
     const float t2 = 1.12f * r7 - 2.0f;
     const float t1s = t1 * t1, t16 = t1s * t1s * t1s, t17 = t16 * t1;
     ev = eps * t17 * t2;
-    const float rd = 1.0f / (r + M_ELD);
+    const float rd = rcp(r + M_ELD);
     ee = M_ELC * qqs * rd;
     if (G) {
       // d t1 / d rho = -1.07 rq^2 = -t1 rq; d t2 / d rho = -7.84 rho^6 r7^2
       const float dv = eps * iR * (-7.0f * t17 * rq * t2 - 7.84f * t17 * rho6 * r7 * r7);
       const float de = -M_ELC * qqs * rd * rd;
-      const float f = (dv + de) / r;
+      const float f = (dv + de) * rcp(r);
       acc.add_pair(i, j, f * d.x, f * d.y, f * d.z);
     }
   }
