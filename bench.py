@@ -53,6 +53,8 @@ def main():
     ap.add_argument("--max-iters", type=int, default=2000)
     ap.add_argument("--grad-tol", type=float, default=1e-4)
     ap.add_argument("--lbfgs", action="store_true")
+    ap.add_argument("--lbfgs-m", type=int, default=10,
+                    help="L-BFGS history length with --lbfgs (the chain configuration uses m = 10)")
     ap.add_argument("--sample", type=int, default=4096, help="conformers checked by the FP64 oracle")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -100,7 +102,7 @@ def main():
     db = engine.device_batch(wl.params, wl.conf_counts)
     x_dev = torch.from_numpy(wl.x).cuda()
     runner = engine.NkRunner(db, max_iters=args.max_iters, grad_tol=args.grad_tol, use_lbfgs=args.lbfgs,
-                             ls_grad=True, precond=pc)
+                             lbfgs_m=args.lbfgs_m, ls_grad=True, precond=pc)
     torch.cuda.synchronize()
     t_pack = time.perf_counter() - t0
     log(f"packed + uploaded in {t_pack:.1f}s; launches "
@@ -138,7 +140,7 @@ def main():
     e_gpu = res.energy.cpu().numpy()
     xg = res.x.cpu().numpy()
     n_status0 = int((st == 0).sum())
-    B = algorithmic_bytes(wl, cnt, dense=not args.lbfgs)
+    B = algorithmic_bytes(wl, cnt, dense=not args.lbfgs, m_hist=args.lbfgs_m)
     achieved = B / (kern_ms * 1e-3)
 
     # ---- cpu_baseline leg: FP64 oracle on a seeded sample (also the Delta-E checker) ----
@@ -166,7 +168,8 @@ def main():
             pcn = pc.cpu().numpy().astype(np.float64)
             h0s = np.concatenate([pcn[off[c] // 3:(off[c] + n3[c]) // 3] for c in pick])
         _, e_ref, st_ref, cnt_ref = oracle.minimize_nk(ob, xs, max_iters=args.max_iters, grad_tol=args.grad_tol,
-                                                       use_lbfgs=args.lbfgs, m_hist=8, n_threads=threads, h0=h0s)
+                                                       use_lbfgs=args.lbfgs, m_hist=args.lbfgs_m, n_threads=threads,
+                                                       h0=h0s)
         t_cpu = time.perf_counter() - t0
         e_gpu_fp64, _, _ = oracle.energy_grad_batch(ob, xgs, family=oracle.FAMILY_TG, n_threads=threads)
         dE = np.abs(e_gpu_fp64 - e_ref)
@@ -206,7 +209,7 @@ def main():
         "dtype": "fp32 terms, fp64 energy sums",
         "data": "synthetic: seeded molecules, typed by the built-in MMFF94-style typer, embedded + N(0,0.1A) noise",
         "config": {"workload": f"{args.workload}: {len(wl.params)} molecules x {wl.conf_counts[0]} conformers",
-                   "algorithm": ("L-BFGS m=8" if args.lbfgs else "dense BFGS")
+                   "algorithm": (f"L-BFGS m={args.lbfgs_m}" if args.lbfgs else "dense BFGS")
                                 + (" from learned H0" if pc is not None else ""), "max_iters": args.max_iters,
                    "grad_tol": args.grad_tol, "atoms_min_max": [int(wl.atoms_per_conf().min()),
                                                                 int(wl.atoms_per_conf().max())],
@@ -220,7 +223,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK, 4),
                      "traffic": None if traffic is None else round(traffic),
                      "traffic_frac": None if traffic is None else round(traffic / (kern_ms * 1e-3) / HBM_PEAK, 4),
-                     "note": (("achieved = algorithmic bytes (16mn+48n per L-BFGS iteration, m = 8, + "
+                     "note": (("achieved = algorithmic bytes (16mn+48n per L-BFGS iteration + "
                                "table/vector bytes per evaluation) of one step / the step's GPU time (HIP events "
                                "around its k_minimize_nk launches); the (s, y) history lives in LDS up to n = 192, "
                                "in global memory (L2 / Infinity Cache resident) above") if args.lbfgs else

@@ -149,6 +149,27 @@ def test_placements_and_blocks_agree(small_wl):
         assert np.median(np.abs(o - outs[0])) < 1e-3
 
 
+@pytest.mark.parametrize("use_lbfgs", [False, True])
+def test_global_hessian_bonded_staging_is_bit_identical(small_wl, use_lbfgs):
+    """Beside a global Hessian (or L-BFGS history) the bonded term tables may be staged in LDS.
+    The arithmetic is the same, so positions, energies, statuses and counters match the launch
+    that reads every table from global memory bit for bit."""
+    eng = _eng()
+    wl = small_wl
+    db = eng.device_batch(wl.params, wl.conf_counts)
+    xd = torch.from_numpy(wl.x).cuda()
+    outs = []
+    for stage in (True, False):
+        r = eng.NkRunner(db, max_iters=2000, grad_tol=1e-4, buckets=((1 << 20, 256, 2),), stage_tables=stage,
+                         use_lbfgs=use_lbfgs, ls_grad=True)
+        res = r.run(xd)
+        torch.cuda.synchronize()
+        outs.append([t.cpu().numpy().copy() for t in (res.x, res.energy, res.status, res.counters)])
+    for a, b in zip(*outs):
+        assert np.array_equal(a, b)
+    assert (outs[0][2] <= 1).all()
+
+
 def test_bit_reproducible(small_wl):
     """Fixed-point gradient sums and fixed-order reductions: identical results run to run."""
     eng = _eng()
