@@ -684,9 +684,12 @@ __device__ __forceinline__ float h_init(const float* pc, float psc, int i, int j
 
 // HM: where the dense inverse Hessian lives -- 0 packed in LDS, 1 full in HBM, 2 packed in HBM
 // TL: term tables read from global memory (0), staged in LDS (1), or bonded tables staged and
-// pairs read from global memory (2)
+// pairs read from global memory (2). Global-Hessian kernels of up to 256 threads are compiled
+// for three waves per SIMD (at most 168 VGPRs), so three workgroups share a CU: their LDS
+// use is small, and the extra resident conformers hide memory latency better than the
+// registers the compiler would otherwise use (medium workload: 172.6 against 178.5 ms).
 template <int ALGO, int HM, int TL, int NT>
-__global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
+__global__ void __launch_bounds__(NT, (HM == 1 && NT <= 256) ? 3 : 1) k_minimize_nk(KBatch B, NkParams P) {
   constexpr bool HLDS = HM == 0;    // LDS-resident (also selects the LDS L-BFGS history)
   constexpr bool PACKED = HM != 1;  // packed upper triangle, select read + column-pair write-back
   extern __shared__ __align__(16) unsigned char smem[];
@@ -1033,9 +1036,10 @@ __global__ void __launch_bounds__(NT) k_minimize_nk(KBatch B, NkParams P) {
           // element is read; when all NPM slots are pending the rows are written back by their
           // only owner, so H is written once every NPM iterations. Adding the updates one after
           // the other rounds each element exactly as writing back after every update would.
-          // HQ rows per lane are loaded together (fewer at large blocks, whose register budget
-          // is smaller).
-          constexpr int HQ = NT >= 640 ? 4 : 8;
+          // HQ rows per lane are loaded together: four fits the three-waves-per-SIMD register
+          // budget (with two waves per SIMD, four and eight rows measured 178.5 and 181.1 ms on
+          // the medium workload).
+          constexpr int HQ = 4;
           const int nq = (n + 3) >> 2;
           int lq = 0;
           while (lq < 3 && (nq << (lq + 1)) <= NT) lq++;
