@@ -314,20 +314,25 @@ __device__ __forceinline__ TermData load_term(const TView& V, int t, int c1, int
   return d;
 }
 
-template <int FAM, bool G, bool PT, int NT>
+// PREF: load each bonded term's tables one step ahead (tables in global memory); with the
+// tables in LDS the loads are short and the prefetch only holds registers.
+template <int FAM, bool G, bool PT, int NT, bool PREF = true>
 __device__ __forceinline__ double eval_terms(const TView& TV, const MolDesc& m, const float* xs,
                                              unsigned long long* gacc, int diel, uint32_t mask, double* et) {
   LdsAcc<G> acc{gacc};
   double e = 0.0;
-  // bonded terms (one generic loop, next term prefetched)
+  // bonded terms (one generic loop, next term prefetched when PREF)
   const int c1 = m.nb, c2 = c1 + m.na, c3 = c2 + m.nsb, c4 = c3 + m.noop, c5 = c4 + m.ntor;
   int t = threadIdx.x;
   TermData cur;
-  if (t < c5) cur = load_term(TV, t, c1, c2, c3, c4, c5);
+  if (PREF && t < c5) cur = load_term(TV, t, c1, c2, c3, c4, c5);
   while (t < c5) {
     const int tn = t + NT;
     TermData nxt;
-    if (tn < c5) nxt = load_term(TV, tn, c1, c2, c3, c4, c5);
+    if (!PREF)
+      cur = load_term(TV, t, c1, c2, c3, c4, c5);
+    else if (tn < c5)
+      nxt = load_term(TV, tn, c1, c2, c3, c4, c5);
     const int i0 = (int)(cur.a & 0xffff), i1 = (int)(cur.a >> 16);
     const int i2 = (int)(cur.b & 0xffff), i3 = (int)(cur.b >> 16);
     if (t < c1) {
@@ -361,7 +366,7 @@ __device__ __forceinline__ double eval_terms(const TView& TV, const MolDesc& m, 
         if (PT) et[4] += v;
       }
     }
-    cur = nxt;
+    if (PREF) cur = nxt;
     t = tn;
   }
   // non-bonded pairs: two pairs per lane per step (independent dependency chains), the
@@ -783,7 +788,7 @@ __global__ void __launch_bounds__(NT, (HM == 1 && NT <= 256) ? 3 : 1) k_minimize
 
     int n_grads = 1, n_evals = 0, iters = 0;
     // initial energy + gradient (mmff_bfgs_source_tg.metal:115-135)
-    double E = R.sum(eval_terms<FAM_TG, true, false, NT>(TV, m, xs, gacc, P.diel, 0x7fu, nullptr));
+    double E = R.sum(eval_terms<FAM_TG, true, false, NT, TL == 0>(TV, m, xs, gacc, P.diel, 0x7fu, nullptr));
     float gscale = GRAD_SCALE_INIT;
     {
       double mxg = 0.0;
@@ -858,9 +863,9 @@ __global__ void __launch_bounds__(NT, (HM == 1 && NT <= 256) ? 3 : 1) k_minimize
         __syncthreads();
         double te;
         if (P.ls_grad)
-          te = R.sum(eval_terms<FAM_TG, true, false, NT>(TV, m, xs, gacc, P.diel, 0x7fu, nullptr));
+          te = R.sum(eval_terms<FAM_TG, true, false, NT, TL == 0>(TV, m, xs, gacc, P.diel, 0x7fu, nullptr));
         else
-          te = R.sum(eval_terms<FAM_TG, false, false, NT>(TV, m, xs, gacc, P.diel, 0x7fu, nullptr));
+          te = R.sum(eval_terms<FAM_TG, false, false, NT, TL == 0>(TV, m, xs, gacc, P.diel, 0x7fu, nullptr));
         n_evals++;
         if (te - E_old <= FUNCTOL * lam * slope) {
           E = te;
@@ -928,7 +933,7 @@ __global__ void __launch_bounds__(NT, (HM == 1 && NT <= 256) ? 3 : 1) k_minimize
       if (!have_grad) {
         for (int i = tid; i < n; i += NT) gacc[i] = 0ull;
         __syncthreads();
-        eval_terms<FAM_TG, true, false, NT>(TV, m, xs, gacc, P.diel, 0x7fu, nullptr);
+        eval_terms<FAM_TG, true, false, NT, TL == 0>(TV, m, xs, gacc, P.diel, 0x7fu, nullptr);
         __syncthreads();
       }
       n_grads++;
