@@ -77,13 +77,15 @@ def table_lds_bytes(hb: HostBatch, pairs: bool = True) -> np.ndarray:
 # Size buckets for the nk minimizer: (max 3N, threads per workgroup, Hessian placement).
 # Each bucket is its own persistent launch sized for its largest molecule. Small molecules
 # keep the packed inverse Hessian in LDS. Above n = 96 the full-storage Hessian in global
-# memory (L2/MALL resident, read as coalesced 16-byte column quads) wins: its workgroups need
-# little LDS, so two 256-thread workgroups share a CU (the register file is the limit) and
-# one's memory waits overlap the other's term evaluation. The LDS-resident triangle of a
-# molecule above n = 96 holds a whole CU's LDS for one workgroup. Measured on MI355X on the
-# medium workload (1,000 x 10): 222 ms per step for these buckets, 305 ms for size-matched
-# LDS buckets up to n = 264.
-DEFAULT_BUCKETS = ((96, 128, _lib.HESS_LDS), (1 << 20, 256, _lib.HESS_GLOBAL))
+# memory (L2 / Infinity Cache resident, read as coalesced 16-byte column quads) wins: its
+# workgroups need little LDS, so three 256-thread workgroups share a CU and one's memory
+# waits overlap the others' term evaluation, where an LDS-resident triangle above n = 96
+# holds a whole CU's LDS for one workgroup. Above n = 360 one 512-thread workgroup per CU
+# does better: three large Hessians per CU outgrow the Infinity Cache. Measured on MI355X:
+# medium workload (1,000 x 10, n <= 330) 173 ms per step against 305 ms for size-matched LDS
+# buckets; ragged workload (200k conformers, n <= 471) 4.89 s against 5.48 s with one global
+# bucket at 256 threads.
+DEFAULT_BUCKETS = ((96, 128, _lib.HESS_LDS), (360, 256, _lib.HESS_GLOBAL), (1 << 20, 512, _lib.HESS_GLOBAL))
 # L-BFGS keeps its (s, y) history in LDS for small molecules and in global memory above
 # n = 192 (256 threads): chain workload subset (250 x 8, n = 450-1200), 855 ms per step
 # against 1071 ms with LDS history up to n = 600.
@@ -204,7 +206,8 @@ TUNE_CLASSES = (
     (96, ((128, _lib.HESS_LDS), (256, _lib.HESS_LDS), (128, _lib.HESS_GLOBAL), (256, _lib.HESS_GLOBAL))),
     (192, ((192, _lib.HESS_LDS), (384, _lib.HESS_LDS), (128, _lib.HESS_GLOBAL), (192, _lib.HESS_GLOBAL), (256, _lib.HESS_GLOBAL))),
     (264, ((640, _lib.HESS_LDS), (192, _lib.HESS_GLOBAL), (256, _lib.HESS_GLOBAL), (320, _lib.HESS_GLOBAL))),
-    (1 << 20, ((192, _lib.HESS_GLOBAL), (256, _lib.HESS_GLOBAL), (512, _lib.HESS_GLOBAL))),
+    (360, ((192, _lib.HESS_GLOBAL), (256, _lib.HESS_GLOBAL), (512, _lib.HESS_GLOBAL))),
+    (1 << 20, ((256, _lib.HESS_GLOBAL), (512, _lib.HESS_GLOBAL), (640, _lib.HESS_GLOBAL))),
 )
 
 
