@@ -9,7 +9,7 @@ its final geometry is within 0.1 kcal/mol of the FP64 oracle minimization from t
 start; the oracle runs on a seeded sample (the cpu_baseline leg), and the within-0.1
 fraction measured there scales the GPU's status-0 count.
 
-Prints one JSON line (see the keys below). --gpus N > 1 exits: this build runs one GPU.
+Prints one JSON line (see the keys below). --gpus N > 1 exits: the package runs on one GPU.
 """
 from __future__ import annotations
 
@@ -67,8 +67,9 @@ def main():
                          "separately seeded 300 x 2 training set)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); "
-              "the minimization shards by molecule with no communication, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py: the package runs on one GPU only (asked for {args.gpus}); "
+              "the minimization would shard by whole molecules, with no communication between GPUs",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -1,6 +1,6 @@
 """GPU tests of the drop-in APIs, restating the reference's own contract tests
 (tests/test_metal_multi_mol_convergence.py:39-93, tests/test_mmff_minimize_nk.py:25-58,
-tests/test_macrocycle_bounds.py:82-88) on this build's molecules and kernels."""
+tests/test_macrocycle_bounds.py:82-88) on the package's own molecules and kernels."""
 from __future__ import annotations
 
 import numpy as np
