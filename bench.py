@@ -96,6 +96,7 @@ def main():
         torch.cuda.synchronize()
         pc_info = {"train": "medium generator, 300 molecules x 2 conformers, seed base + 1, BFGS-minimized",
                    "target": "diagonal 3x3 blocks of the inverse FD Hessian at the minima",
+                   "start_scale": precond.START_SCALE,
                    "loss_first_last": [round(lg.losses[0], 4), round(lg.losses[-1], 4)],
                    "fit_s": round(time.perf_counter() - t0, 1)}
         log(f"learned preconditioner fitted in {pc_info['fit_s']}s, loss {pc_info['loss_first_last']}")

@@ -257,12 +257,24 @@ def train(model: PrecondMLP, feat, target, weight, steps: int = 400, lr: float =
     return TrainLog(losses=losses, n_atoms=n)
 
 
+# Multiplier on the network's blocks when they start a minimization. The blocks approximate
+# diagonal blocks of the inverse Hessian at a minimum; from a start geometry the full step they
+# imply overshoots and the line search pays for it. Measured on the medium workload (1,000 x
+# 10, network trained on a separate 300 x 2 set):
+#   scale 1.0: 311.6 mean iterations, 1.76 evaluations per iteration, 176 ms per step
+#   scale 0.2: 299.3 mean iterations, 1.37 evaluations per iteration, 158 ms per step
+#   plain BFGS: 320.9 mean iterations, 1.40 evaluations per iteration, 173 ms per step
+# (an identity start scaled by 0.1 gives 299.0, 1.46 and 166 ms).
+START_SCALE = 0.2
+
+
 def precond_for_batch(model: PrecondMLP, mols: Sequence, params_list: Sequence, conf_counts: Sequence[int],
-                      device: str = "cuda"):
-    """(sum_c N_c, 9) device tensor in the layout NkRunner(precond=...) expects."""
+                      device: str = "cuda", scale: float = START_SCALE):
+    """(sum_c N_c, 9) device tensor in the layout NkRunner(precond=...) expects: the network's
+    blocks times ``scale``."""
     torch = _lib.require_gpu()
     feat = torch.from_numpy(conformer_features(mols, params_list, conf_counts)).to(device)
-    return model.blocks(feat)
+    return (model.blocks(feat) * float(scale)).contiguous()
 
 
 def fit(train_wl, steps: int = 800, lr: float = 3e-3, batch: int = 8192, seed: int = 0, target: str = "block-of-inverse",
