@@ -130,3 +130,27 @@ def test_pack_layout_and_validation():
     bad.torsion_idx4[0] = 99
     with pytest.raises(ValueError):
         packing.pack([bad], [1])
+
+
+def test_bucket_tables_cover_every_size_and_name_known_placements():
+    """The default and L-BFGS bucket tables ascend and end with an unbounded class; every
+    autotuner candidate names a block the planner accepts and a known placement. The staged
+    bonded-table size is the full staging size without the pair tables."""
+    from mlxmolkit_archive_amd import _lib, engine
+    places = {_lib.HESS_AUTO, _lib.HESS_LDS, _lib.HESS_GLOBAL, _lib.HESS_GLOBAL_PACKED}
+    for table in (engine.DEFAULT_BUCKETS, engine.LBFGS_BUCKETS):
+        bounds = [b for b, _, _ in table]
+        assert bounds == sorted(bounds) and bounds[-1] >= 3 * 65535
+        for _, block, place in table:
+            assert block % 64 == 0 and 64 <= block <= 1024 and place in places
+    bounds = [b for b, _ in engine.TUNE_CLASSES]
+    assert bounds == sorted(bounds) and bounds[-1] >= 3 * 65535
+    for _, cands in engine.TUNE_CLASSES:
+        for block, place in cands:
+            assert block % 64 == 0 and 64 <= block <= 1024 and place in places
+    params = [_mol(smi)[1] for smi in ("CCO", "c1ccccc1", "CC(=O)Nc1ccccc1", "CCCCCC")]
+    hb = packing.pack(params, [1] * len(params))
+    full = engine.table_lds_bytes(hb)
+    bonded = engine.table_lds_bytes(hb, pairs=False)
+    npair = hb.mol_desc.reshape(-1, 16)[:, 12].astype(np.int64)
+    assert np.array_equal(full - bonded, 20 * npair)
