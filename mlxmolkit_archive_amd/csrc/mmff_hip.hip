@@ -514,6 +514,21 @@ __device__ __forceinline__ void hquad_rows(float4* base, int nq, int j, int j1, 
   }
 }
 
+// hquad_rows for the run-time count of pending updates (0..NPM)
+template <int HQ, int NPM, int NU = NPM>
+__device__ __forceinline__ void hquad_dispatch(int npend, float4* base, int nq, int j, int j1, const float2* VG,
+                                               const float2* const (&S)[NPM], const float (&cx)[NPM][4],
+                                               const float (&cy)[NPM][4], float (&a1)[4], float (&a2)[4]) {
+  if constexpr (NU == 0) {
+    hquad_rows<HQ, 0, NPM>(base, nq, j, j1, VG, S, cx, cy, a1, a2);
+  } else {
+    if (npend == NU)
+      hquad_rows<HQ, NU, NPM>(base, nq, j, j1, VG, S, cx, cy, a1, a2);
+    else
+      hquad_dispatch<HQ, NPM, NU - 1>(npend, base, nq, j, j1, VG, S, cx, cy, a1, a2);
+  }
+}
+
 // Row groups [ka, kb) (rows 4k..4k+3) of one column of the padded packed matrix gain
 // alpha_i s + beta_i h (AB[i] = {alpha_i, beta_i}, u = {s, h} of the column); rows past the
 // column's end are padding and never read as matrix elements.
@@ -629,7 +644,7 @@ constexpr float GRAD_SCALE_INIT = 0.1f;
 constexpr double GRAD_CAP = 10.0;
 constexpr int LBFGS_MAX_M = 16;
 // pending rank-2 updates the global-Hessian BFGS pass applies on the fly before writing H back
-constexpr int HPEND_MAX = 2;
+constexpr int HPEND_MAX = 4;
 
 struct NkParams {
   const float* x_in;
@@ -1068,17 +1083,7 @@ __global__ void __launch_bounds__(NT, (HM == 1 && NT <= 256) ? 3 : 1) k_minimize
             }
             float a1[4] = {0.f, 0.f, 0.f, 0.f}, a2[4] = {0.f, 0.f, 0.f, 0.f};
             float4* base = (float4*)H + tq;  // row j of the quad at base[j * nq]
-            static_assert(NPM <= 4, "at most four pending updates");
-            if (npend == 4 && NPM >= 4)
-              hquad_rows<HQ, NPM >= 4 ? 4 : 0, NPM>(base, nq, j0, j1, VG, S, cx, cy, a1, a2);
-            else if (npend == 3 && NPM >= 3)
-              hquad_rows<HQ, NPM >= 3 ? 3 : 0, NPM>(base, nq, j0, j1, VG, S, cx, cy, a1, a2);
-            else if (npend == 2 && NPM >= 2)
-              hquad_rows<HQ, NPM >= 2 ? 2 : 0, NPM>(base, nq, j0, j1, VG, S, cx, cy, a1, a2);
-            else if (npend == 1)
-              hquad_rows<HQ, 1, NPM>(base, nq, j0, j1, VG, S, cx, cy, a1, a2);
-            else
-              hquad_rows<HQ, 0, NPM>(base, nq, j0, j1, VG, S, cx, cy, a1, a2);
+            hquad_dispatch<HQ, NPM>(npend, base, nq, j0, j1, VG, S, cx, cy, a1, a2);
 #pragma unroll
             for (int u = 0; u < 4; u++) {
               float s1 = a1[u], s2 = a2[u];
