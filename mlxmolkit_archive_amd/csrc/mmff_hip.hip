@@ -482,35 +482,63 @@ __device__ __forceinline__ void hquad_dot(const float4& h, float2 v, float (&a1)
   a2[3] = fmaf(h.w, v.y, a2[3]);
 }
 
-// Rows [j, j1) of a column quad of the full Hessian (row r at base[r * nq]): apply the NU
-// oldest of the pending rank-2 updates in order (update k: row {s, h} in S[k], column
-// coefficients cx[k], cy[k]), accumulate H v for v = {dg, g}, and write the rows back when
-// all NPM slots were applied. HQ rows are loaded together.
+// Rows [j, j1) of a column quad of the full Hessian (row r at base[r * nq]): accumulate
+// H v for v = {dg, g} into a1, a2, with H the stored rows plus the NU oldest pending rank-2
+// updates (update k: row vectors {s, h} in S[k], column coefficients cx[k], cy[k]). When all
+// NPM slots are pending the updates are added to each element, oldest first, and the rows
+// are written back. Otherwise the stored rows are only read, and the updates' part of H v is
+// left as four dot products per update over these rows in lr (s.dg, h.dg, s.g, h.g), which
+// the caller combines with the column coefficients after summing the quad's lanes. HQ rows
+// are loaded together.
 template <int HQ, int NU, int NPM>
 __device__ __forceinline__ void hquad_rows(float4* base, int nq, int j, int j1, const float2* VG,
                                            const float2* const (&S)[NPM], const float (&cx)[NPM][4],
-                                           const float (&cy)[NPM][4], float (&a1)[4], float (&a2)[4]) {
+                                           const float (&cy)[NPM][4], float (&a1)[4], float (&a2)[4],
+                                           float (&lr)[4][NPM]) {
+  constexpr bool FLUSH = NU == NPM;
   for (; j + HQ - 1 < j1; j += HQ) {
     float4 h[HQ];
 #pragma unroll
     for (int u = 0; u < HQ; u++) h[u] = base[(size_t)(j + u) * nq];
 #pragma unroll
     for (int u = 0; u < HQ; u++) {
+      const float2 v = VG[j + u];
 #pragma unroll
-      for (int k = 0; k < NU; k++) hquad_upd(h[u], S[k][j + u], cx[k], cy[k]);
-      hquad_dot(h[u], VG[j + u], a1, a2);
+      for (int k = 0; k < NU; k++) {
+        const float2 w = S[k][j + u];
+        if (FLUSH) {
+          hquad_upd(h[u], w, cx[k], cy[k]);
+        } else {
+          lr[0][k] = fmaf(w.x, v.x, lr[0][k]);
+          lr[1][k] = fmaf(w.y, v.x, lr[1][k]);
+          lr[2][k] = fmaf(w.x, v.y, lr[2][k]);
+          lr[3][k] = fmaf(w.y, v.y, lr[3][k]);
+        }
+      }
+      hquad_dot(h[u], v, a1, a2);
     }
-    if (NU == NPM) {
+    if (FLUSH) {
 #pragma unroll
       for (int u = 0; u < HQ; u++) base[(size_t)(j + u) * nq] = h[u];
     }
   }
   for (; j < j1; j++) {
     float4 h = base[(size_t)j * nq];
+    const float2 v = VG[j];
 #pragma unroll
-    for (int k = 0; k < NU; k++) hquad_upd(h, S[k][j], cx[k], cy[k]);
-    hquad_dot(h, VG[j], a1, a2);
-    if (NU == NPM) base[(size_t)j * nq] = h;
+    for (int k = 0; k < NU; k++) {
+      const float2 w = S[k][j];
+      if (FLUSH) {
+        hquad_upd(h, w, cx[k], cy[k]);
+      } else {
+        lr[0][k] = fmaf(w.x, v.x, lr[0][k]);
+        lr[1][k] = fmaf(w.y, v.x, lr[1][k]);
+        lr[2][k] = fmaf(w.x, v.y, lr[2][k]);
+        lr[3][k] = fmaf(w.y, v.y, lr[3][k]);
+      }
+    }
+    hquad_dot(h, v, a1, a2);
+    if (FLUSH) base[(size_t)j * nq] = h;
   }
 }
 
@@ -518,14 +546,15 @@ __device__ __forceinline__ void hquad_rows(float4* base, int nq, int j, int j1, 
 template <int HQ, int NPM, int NU = NPM>
 __device__ __forceinline__ void hquad_dispatch(int npend, float4* base, int nq, int j, int j1, const float2* VG,
                                                const float2* const (&S)[NPM], const float (&cx)[NPM][4],
-                                               const float (&cy)[NPM][4], float (&a1)[4], float (&a2)[4]) {
+                                               const float (&cy)[NPM][4], float (&a1)[4], float (&a2)[4],
+                                               float (&lr)[4][NPM]) {
   if constexpr (NU == 0) {
-    hquad_rows<HQ, 0, NPM>(base, nq, j, j1, VG, S, cx, cy, a1, a2);
+    hquad_rows<HQ, 0, NPM>(base, nq, j, j1, VG, S, cx, cy, a1, a2, lr);
   } else {
     if (npend == NU)
-      hquad_rows<HQ, NU, NPM>(base, nq, j, j1, VG, S, cx, cy, a1, a2);
+      hquad_rows<HQ, NU, NPM>(base, nq, j, j1, VG, S, cx, cy, a1, a2, lr);
     else
-      hquad_dispatch<HQ, NPM, NU - 1>(npend, base, nq, j, j1, VG, S, cx, cy, a1, a2);
+      hquad_dispatch<HQ, NPM, NU - 1>(npend, base, nq, j, j1, VG, S, cx, cy, a1, a2, lr);
   }
 }
 
@@ -1052,10 +1081,12 @@ __global__ void __launch_bounds__(NT, (HM == 1 && NT <= 256) ? 3 : 1) k_minimize
           // full storage in HBM with rows padded to ldh = round4(n) floats (16-byte aligned).
           // 2^lq consecutive lanes share the column quad 4tq..4tq+3 and split its rows into
           // equal segments; a row of a quad is one 16-byte load and consecutive quads are
-          // consecutive addresses (coalesced). Pending updates are added, oldest first, as each
-          // element is read; when all NPM slots are pending the rows are written back by their
-          // only owner, so H is written once every NPM iterations. Adding the updates one after
-          // the other rounds each element exactly as writing back after every update would.
+          // consecutive addresses (coalesced). H is written once every NPM iterations: when all
+          // NPM update slots are pending, they are added to each element as it is read, oldest
+          // first (which rounds each stored element exactly as a write-back after every update
+          // would), and the rows are written back by their only owner. In the other iterations
+          // the stored rows are only read and the pending updates enter H v through their
+          // factored form, alpha_t (s.v) + beta_t (h.v), from dot products the same pass takes.
           // HQ rows per lane are loaded together: four fits the three-waves-per-SIMD register
           // budget (with two waves per SIMD, four and eight rows measured 178.5 and 181.1 ms on
           // the medium workload).
@@ -1083,7 +1114,24 @@ __global__ void __launch_bounds__(NT, (HM == 1 && NT <= 256) ? 3 : 1) k_minimize
             }
             float a1[4] = {0.f, 0.f, 0.f, 0.f}, a2[4] = {0.f, 0.f, 0.f, 0.f};
             float4* base = (float4*)H + tq;  // row j of the quad at base[j * nq]
-            hquad_dispatch<HQ, NPM>(npend, base, nq, j0, j1, VG, S, cx, cy, a1, a2);
+            float lr[4][NPM];
+#pragma unroll
+            for (int k = 0; k < NPM; k++) lr[0][k] = lr[1][k] = lr[2][k] = lr[3][k] = 0.f;
+            hquad_dispatch<HQ, NPM>(npend, base, nq, j0, j1, VG, S, cx, cy, a1, a2, lr);
+            // the pending updates' part of H v when they were not added element by element:
+            // sum the dot products over the quad's lanes, then y_t += alpha_t (s.v) + beta_t (h.v)
+            if (npend > 0 && npend < NPM) {
+#pragma unroll
+              for (int k = 0; k < NPM; k++)
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                  float x = lr[c][k];
+                  if (lq >= 1) x += dpp_f<DPP_QUAD_XOR1>(x);
+                  if (lq >= 2) x += dpp_f<DPP_QUAD_XOR2>(x);
+                  if (lq >= 3) x += dpp_f<DPP_ROW_HALF_MIRROR>(x);
+                  lr[c][k] = x;
+                }
+            }
 #pragma unroll
             for (int u = 0; u < 4; u++) {
               float s1 = a1[u], s2 = a2[u];
@@ -1098,6 +1146,13 @@ __global__ void __launch_bounds__(NT, (HM == 1 && NT <= 256) ? 3 : 1) k_minimize
               if (lq >= 3) {
                 s1 += dpp_f<DPP_ROW_HALF_MIRROR>(s1);
                 s2 += dpp_f<DPP_ROW_HALF_MIRROR>(s2);
+              }
+              if (npend > 0 && npend < NPM) {
+#pragma unroll
+                for (int k = 0; k < NPM; k++) {
+                  s1 = fmaf(cy[k][u], lr[1][k], fmaf(cx[k][u], lr[0][k], s1));
+                  s2 = fmaf(cy[k][u], lr[3][k], fmaf(cx[k][u], lr[2][k], s2));
+                }
               }
               const int t = 4 * tq + u;
               if (q == 0 && t < n) {
