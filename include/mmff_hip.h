@@ -16,6 +16,7 @@
  *   mmffhip_bb_fused        <- mmff_optimize_metal_fused (mlxmolkit/mmff_metal_kernel.py:1104-1150)
  *   mmffhip_multimol_*      <- the per-iteration mega-kernel launches + MLX L-BFGS of
  *                              mmff_optimize_metal_multi_mol (mlxmolkit/mmff_metal_optimizer.py:307-495)
+ *                              and of mmff_optimize_metal_batch (:77-202)
  */
 #ifndef MMFF_HIP_H
 #define MMFF_HIP_H
@@ -144,6 +145,17 @@ int mmffhip_multimol_run(const mmffhip_batch *b, const float *x_in, float *x_bes
                          int32_t max_iters, int32_t eval_freq, float max_step_per_atom, int32_t lbfgs_m,
                          float grad_scale, float grad_tol, void *workspace, size_t workspace_bytes,
                          void *stream);
+
+/* The same engine with flags. MMFFHIP_MM_BATCH gives mmff_optimize_metal_batch
+ * (mlxmolkit/mmff_metal_optimizer.py:77-202): max_iters iterations with the best-x guard at
+ * every sync point and no convergence exit, no polish pass, and grad_norm = ||g_sc||_2 of the
+ * last iterate (:186); per_conf_iters = max_iters, converged = 0, iters_used = max_iters. */
+#define MMFFHIP_MM_BATCH 1
+int mmffhip_multimol_run_ex(const mmffhip_batch *b, const float *x_in, float *x_best, double *e_final,
+                            float *grad_norm, int32_t *per_conf_iters, int32_t *converged, int32_t *iters_used,
+                            int32_t max_iters, int32_t eval_freq, float max_step_per_atom, int32_t lbfgs_m,
+                            float grad_scale, float grad_tol, int32_t flags, void *workspace,
+                            size_t workspace_bytes, void *stream);
 
 /* Learned initial inverse Hessian: a per-atom MLP (104 features -> 64 -> 64 -> 6, ReLU) whose 6
  * outputs are the Cholesky factor of a 3x3 SPD block (diagonal through softplus + 1e-3). No

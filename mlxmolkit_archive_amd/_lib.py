@@ -19,6 +19,7 @@ FAMILY_MEGA = 1
 BFGS = 0
 LBFGS = 1
 HESS_AUTO, HESS_LDS, HESS_GLOBAL, HESS_GLOBAL_PACKED = 0, 1, 2, 3
+MM_BATCH = 1  # mmffhip_multimol_run_ex flag: the mmff_optimize_metal_batch form
 
 _vp = C.c_void_p
 
@@ -52,6 +53,8 @@ SIGNATURES = {
     "mmffhip_multimol_workspace": (C.c_int, [_vp, C.c_int32, _vp]),
     "mmffhip_multimol_run": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int32, C.c_int32, C.c_float,
                                        C.c_int32, C.c_float, C.c_float, _vp, C.c_size_t, _vp]),
+    "mmffhip_multimol_run_ex": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int32, C.c_int32, C.c_float,
+                                          C.c_int32, C.c_float, C.c_float, C.c_int32, _vp, C.c_size_t, _vp]),
     "mmffhip_mlp_dims": (C.c_int, [_vp, _vp, _vp, _vp]),
     "mmffhip_mlp_infer": (C.c_int, [_vp, _vp, C.c_int32, _vp, _vp]),
     "mmffhip_mlp_workspace": (C.c_int, [C.c_int32, _vp]),

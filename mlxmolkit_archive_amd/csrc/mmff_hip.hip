@@ -1557,10 +1557,12 @@ __global__ void __launch_bounds__(NT) k_mm_chunk(KBatch B, MMState S, int steps,
   }
 }
 
+// l2_norm: grad_norm = ||g_sc||_2 of the last iterate (mmff_optimize_metal_batch,
+// mmff_metal_optimizer.py:186) instead of its largest per-atom norm (multi_mol, :504-525)
 template <int NT>
 __global__ void __launch_bounds__(NT) k_mm_final(KBatch B, MMState S, float* x_best, double* e_final,
                                                  float* grad_norm, int32_t* pci, int32_t* conv, int32_t* iters_used,
-                                                 int n_main_flags, int eval_freq, int max_iters) {
+                                                 int n_main_flags, int eval_freq, int max_iters, int l2_norm) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int NW = NT / 64;
   const int c = blockIdx.x;
@@ -1576,11 +1578,16 @@ __global__ void __launch_bounds__(NT) k_mm_final(KBatch B, MMState S, float* x_b
   Reducer<NT> R{red, 0};
   double E = R.sum(eval_terms<FAM_MEGA, false, false, NT>(global_view(B, m), m, xs, gacc, 1, 0x7fu, nullptr));
   double gmax = 0.0;
-  for (int a = threadIdx.x; a < m.n_atoms; a += NT) {
-    const float gx = S.g[off + 3 * a], gy = S.g[off + 3 * a + 1], gz = S.g[off + 3 * a + 2];
-    gmax = fmax(gmax, (double)sqrtf(gx * gx + gy * gy + gz * gz));
+  if (l2_norm) {
+    for (int i = threadIdx.x; i < n; i += NT) gmax += (double)S.g[off + i] * S.g[off + i];
+    gmax = sqrt(R.sum(gmax));
+  } else {
+    for (int a = threadIdx.x; a < m.n_atoms; a += NT) {
+      const float gx = S.g[off + 3 * a], gy = S.g[off + 3 * a + 1], gz = S.g[off + 3 * a + 2];
+      gmax = fmax(gmax, (double)sqrtf(gx * gx + gy * gy + gz * gz));
+    }
+    gmax = R.max(gmax);
   }
-  gmax = R.max(gmax);
   for (int i = threadIdx.x; i < n; i += NT) x_best[off + i] = xs[i];
   if (threadIdx.x == 0) {
     e_final[c] = E;
@@ -1911,6 +1918,16 @@ int mmffhip_multimol_run(const mmffhip_batch* b, const float* x_in, float* x_bes
                          int32_t* per_conf_iters, int32_t* converged, int32_t* iters_used, int32_t max_iters,
                          int32_t eval_freq, float max_step_per_atom, int32_t lbfgs_m, float grad_scale,
                          float grad_tol, void* workspace, size_t workspace_bytes, void* stream) {
+  return mmffhip_multimol_run_ex(b, x_in, x_best, e_final, grad_norm, per_conf_iters, converged, iters_used, max_iters,
+                                 eval_freq, max_step_per_atom, lbfgs_m, grad_scale, grad_tol, 0, workspace,
+                                 workspace_bytes, stream);
+}
+
+int mmffhip_multimol_run_ex(const mmffhip_batch* b, const float* x_in, float* x_best, double* e_final,
+                            float* grad_norm, int32_t* per_conf_iters, int32_t* converged, int32_t* iters_used,
+                            int32_t max_iters, int32_t eval_freq, float max_step_per_atom, int32_t lbfgs_m,
+                            float grad_scale, float grad_tol, int32_t flags, void* workspace, size_t workspace_bytes,
+                            void* stream) {
   int rc = check_batch(b);
   if (rc) return rc;
   if (b->n_confs == 0) return MMFFHIP_OK;
@@ -1921,8 +1938,10 @@ int mmffhip_multimol_run(const mmffhip_batch* b, const float* x_in, float* x_bes
   rc = mmffhip_multimol_workspace(b, lbfgs_m, &need);
   if (rc) return rc;
   if (workspace_bytes < need) return fail(MMFFHIP_ERR_SIZE, "workspace too small");
+  if (flags & ~(int32_t)MMFFHIP_MM_BATCH) return fail(MMFFHIP_ERR_ARG, "unknown multimol flags");
+  const bool batch = (flags & MMFFHIP_MM_BATCH) != 0;
   const int n_main = (max_iters + eval_freq - 1) / eval_freq;
-  const int polish = max_iters / 4;
+  const int polish = batch ? 0 : max_iters / 4;
   const int n_pol = (polish + eval_freq - 1) / eval_freq;
   if (n_main + n_pol > 4096) return fail(MMFFHIP_ERR_ARG, "too many sync points (max_iters / eval_freq > 4096)");
   const size_t T = (size_t)b->n_confs * 3 * b->max_atoms;
@@ -1968,14 +1987,17 @@ int mmffhip_multimol_run(const mmffhip_batch* b, const float* x_in, float* x_bes
     int steps = std::min(eval_freq, max_iters - it);
     it += steps;
     const bool sync = (it % eval_freq) == 0;
+    // the batch form has no convergence exit: no conformer ever passes the test (tolerance
+    // -1) and no launch reads the previous sync point's count
     hipLaunchKernelGGL((k_mm_chunk<NT>), dim3(b->n_confs), dim3(NT), lds_chunk, s, kb, S, steps, it, sync ? 1 : 0,
-                       0, k > 0 ? S.flags + (k - 1) : (const int32_t*)nullptr, S.flags + k, (int)lbfgs_m,
-                       max_step_per_atom, grad_scale, grad_tol);
+                       0, (k > 0 && !batch) ? S.flags + (k - 1) : (const int32_t*)nullptr, S.flags + k,
+                       (int)lbfgs_m, max_step_per_atom, grad_scale, batch ? -1.0f : grad_tol);
     if (sync) k++;
   }
   HIP_TRY(hipGetLastError());
-  hipLaunchKernelGGL((k_mm_init<NT>), dim3(b->n_confs), dim3(NT), lds_init, s, kb, S, x_in, grad_scale, 1,
-                     (int)max_iters);
+  if (!batch)
+    hipLaunchKernelGGL((k_mm_init<NT>), dim3(b->n_confs), dim3(NT), lds_init, s, kb, S, x_in, grad_scale, 1,
+                       (int)max_iters);
   it = 0;
   int kp = n_main;
   while (it < polish) {
@@ -1990,8 +2012,8 @@ int mmffhip_multimol_run(const mmffhip_batch* b, const float* x_in, float* x_bes
   HIP_TRY(hipGetLastError());
   (void)nfl;
   hipLaunchKernelGGL((k_mm_final<NT>), dim3(b->n_confs), dim3(NT), lds_init, s, kb, S, x_best, e_final, grad_norm,
-                     per_conf_iters, converged, iters_used, (int)(max_iters / eval_freq), (int)eval_freq,
-                     (int)max_iters);
+                     per_conf_iters, converged, iters_used, batch ? 0 : (int)(max_iters / eval_freq),
+                     (int)eval_freq, (int)max_iters, batch ? 1 : 0);
   HIP_TRY(hipGetLastError());
   return MMFFHIP_OK;
 }

@@ -272,8 +272,10 @@ def bb_fused(db: DeviceBatch, x, max_iters=500, max_step=0.3, grad_lr=0.1, strea
 
 
 def multimol(db: DeviceBatch, x, max_iters=200, eval_freq=10, max_step_per_atom=0.3, lbfgs_m=10, grad_scale=0.1,
-             grad_tol=1e-4, stream=None):
-    """Lock-step L-BFGS of mmff_optimize_metal_multi_mol, fully on the device."""
+             grad_tol=1e-4, stream=None, batch_mode=False):
+    """Lock-step L-BFGS of mmff_optimize_metal_multi_mol, fully on the device. ``batch_mode``
+    runs the mmff_optimize_metal_batch form instead: every iteration up to max_iters, no polish
+    pass, grad_norm = ||g_sc||_2 of the last iterate."""
     torch = _lib.require_gpu()
     L = _lib.load()
     nbytes = C.c_size_t()
@@ -287,8 +289,9 @@ def multimol(db: DeviceBatch, x, max_iters=200, eval_freq=10, max_step_per_atom=
     pci = torch.empty(C_, dtype=torch.int32, device=dev)
     conv = torch.empty(C_, dtype=torch.int32, device=dev)
     used = torch.zeros(1, dtype=torch.int32, device=dev)
-    _lib.check(L.mmffhip_multimol_run(C.byref(db.desc), _p(x), _p(xb), _p(e), _p(gn), _p(pci), _p(conv), _p(used),
-                                      int(max_iters), int(eval_freq), float(max_step_per_atom), int(lbfgs_m),
-                                      float(grad_scale), float(grad_tol), _p(ws), int(nbytes.value),
-                                      _lib.stream_ptr(stream)))
+    _lib.check(L.mmffhip_multimol_run_ex(C.byref(db.desc), _p(x), _p(xb), _p(e), _p(gn), _p(pci), _p(conv),
+                                         _p(used), int(max_iters), int(eval_freq), float(max_step_per_atom),
+                                         int(lbfgs_m), float(grad_scale), float(grad_tol),
+                                         _lib.MM_BATCH if batch_mode else 0, _p(ws), int(nbytes.value),
+                                         _lib.stream_ptr(stream)))
     return dict(x=xb, energy=e, grad_norm=gn, per_conf_iters=pci, converged=conv, iters_used=used)

@@ -8,7 +8,10 @@
   whose early exit is decided on the GPU (no host round trip per sync).
 * ``mmff_optimize_metal_fused_multi_mol`` (:543-653): one launch of BB steepest descent;
   ``grad_norms = 0`` and ``n_iters = max_iters`` as in the reference (:646-647).
-* ``mmff_optimize_metal_batch`` (:77-202): single molecule, same engine as multi_mol.
+* ``mmff_optimize_metal_batch`` (:77-202): single molecule, the same device engine in its
+  batch form: every one of ``max_iters`` iterations runs (no convergence exit), no polish
+  pass, best-x guard at every sync point, ``grad_norms`` = ||g_sc||_2 of the last iterate
+  (:186).
 
 Conformers of the caller's molecules are written back in place (:514-519), vectorised per
 conformer through ``SetPositions`` when available.
@@ -139,10 +142,17 @@ def mmff_optimize_metal_fused_multi_mol(mol_specs, max_iters: int = 500, max_ste
 def mmff_optimize_metal_batch(mol, conf_ids=None, max_iters: int = 200, eval_freq: int = 10,
                               max_step_per_atom: float = 0.3, lbfgs_m: int = 10,
                               grad_scale: float = 0.1) -> MetalBatchResult:
-    """Single molecule, many conformers (reference :77-202); shares the multi_mol engine."""
-    res = mmff_optimize_metal_multi_mol([(mol, conf_ids)], max_iters=max_iters, eval_freq=eval_freq,
-                                        max_step_per_atom=max_step_per_atom, lbfgs_m=lbfgs_m,
-                                        grad_scale=grad_scale, grad_tol=0.0)
-    r = res.mol_results[0]
-    return MetalBatchResult(energies=r.energies, positions=r.positions, grad_norms=r.grad_norms, n_iters=max_iters,
-                            n_conformers=r.n_conformers)
+    """Single molecule, many conformers (reference :77-202): L-BFGS for exactly ``max_iters``
+    iterations with the best-x guard every ``eval_freq``, the best-seen positions written back
+    to the conformers, and ``grad_norms`` = ||g_sc||_2 of the last iterate."""
+    torch = _lib.require_gpu()
+    mols, all_ids, params, counts, x_np = _prepare([(mol, conf_ids)])
+    db = device_batch(params, counts)
+    x = torch.from_numpy(x_np).cuda()
+    r = multimol(db, x, max_iters=max_iters, eval_freq=eval_freq, max_step_per_atom=max_step_per_atom,
+                 lbfgs_m=lbfgs_m, grad_scale=grad_scale, grad_tol=0.0, batch_mode=True)
+    torch.cuda.synchronize()
+    e = r["energy"].cpu().numpy().astype(np.float32)
+    res = _unpack(mols, all_ids, params, counts, r["x"].cpu().numpy(), e, r["grad_norm"].cpu().numpy(), max_iters,
+                  None, None)
+    return res.mol_results[0]

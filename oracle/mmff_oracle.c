@@ -1197,9 +1197,20 @@ static void mm_run(mm_job *J, int n_steps, int n_threads) {
     for (int t = 0; t < n_threads; t++) pthread_join(th[t], NULL);
 }
 
-int oracle_multi_mol(const oracle_mol *mols, const int32_t *conf_mol, const int64_t *conf_off, int n_confs,
-                     double *x, int max_iters, int eval_freq, double max_step, int M, double gsc, double tol,
-                     double *e_out, double *gnorm_out, int32_t *per_conf_iters, int32_t *conv, int n_threads) {
+static double mm_l2_gnorm(const mm_state *s) {
+    double a = 0.0;
+    for (int k = 0; k < s->n; k++) a += s->g[k] * s->g[k];
+    return sqrt(a);
+}
+
+/* flags: ORACLE_MM_BATCH selects mmff_optimize_metal_batch (mmff_metal_optimizer.py:77-202):
+ * max_iters steps with the best-x guard at every sync point and no convergence exit, no
+ * polish pass, and gnorm_out = ||g_sc|| of the last iterate (:186). */
+int oracle_multi_mol_ex(const oracle_mol *mols, const int32_t *conf_mol, const int64_t *conf_off, int n_confs,
+                        double *x, int max_iters, int eval_freq, double max_step, int M, double gsc, double tol,
+                        double *e_out, double *gnorm_out, int32_t *per_conf_iters, int32_t *conv, int n_threads,
+                        int flags) {
+    const int batch = (flags & ORACLE_MM_BATCH) != 0;
     mm_state *st = calloc((size_t)n_confs, sizeof(mm_state));
     for (int c = 0; c < n_confs; c++) {
         mm_state *s = &st[c];
@@ -1251,20 +1262,21 @@ int oracle_multi_mol(const oracle_mol *mols, const int32_t *conf_mol, const int6
                 mm_state *s = &st[c];
                 if (s->e_new < s->best_e) memcpy(s->best_x, s->x, sizeof(double) * (size_t)s->n);
                 s->best_e = dmin(s->e_new, s->best_e);
+                if (batch) continue;
                 if (!conv[c] && mm_max_atom_gnorm(s) <= tol) {
                     per_conf_iters[c] = it;
                     conv[c] = 1;
                 }
                 all &= conv[c];
             }
-            if (all) {
+            if (all && !batch) {
                 iters_used = it;
                 break;
             }
         }
     }
-    /* polish, :418-481 */
-    for (int c = 0; c < n_confs; c++) {
+    /* polish, :418-481 (not in the batch form) */
+    for (int c = 0; c < n_confs && !batch; c++) {
         mm_state *s = &st[c];
         const oracle_mol *m = &mols[conf_mol[c]];
         memcpy(s->x, s->best_x, sizeof(double) * (size_t)s->n);
@@ -1273,7 +1285,7 @@ int oracle_multi_mol(const oracle_mol *mols, const int32_t *conf_mol, const int6
         s->len = 0;
         s->head = 0;
     }
-    int polish = max_iters / 4;
+    int polish = batch ? 0 : max_iters / 4;
     it = 0;
     while (it < polish) {
         int steps = eval_freq - (it % eval_freq);
@@ -1297,7 +1309,7 @@ int oracle_multi_mol(const oracle_mol *mols, const int32_t *conf_mol, const int6
         const oracle_mol *m = &mols[conf_mol[c]];
         memcpy(x + conf_off[c], s->best_x, sizeof(double) * (size_t)s->n);
         e_out[c] = oracle_energy_grad(m, ORACLE_FAMILY_MEGA, 1, s->best_x, NULL, T_ALL, NULL);
-        gnorm_out[c] = mm_max_atom_gnorm(s);
+        gnorm_out[c] = batch ? mm_l2_gnorm(s) : mm_max_atom_gnorm(s);
         free(s->x);
         free(s->g);
         free(s->best_x);
@@ -1312,4 +1324,11 @@ int oracle_multi_mol(const oracle_mol *mols, const int32_t *conf_mol, const int6
     }
     free(st);
     return iters_used;
+}
+
+int oracle_multi_mol(const oracle_mol *mols, const int32_t *conf_mol, const int64_t *conf_off, int n_confs,
+                     double *x, int max_iters, int eval_freq, double max_step, int M, double gsc, double tol,
+                     double *e_out, double *gnorm_out, int32_t *per_conf_iters, int32_t *conv, int n_threads) {
+    return oracle_multi_mol_ex(mols, conf_mol, conf_off, n_confs, x, max_iters, eval_freq, max_step, M, gsc, tol,
+                               e_out, gnorm_out, per_conf_iters, conv, n_threads, 0);
 }

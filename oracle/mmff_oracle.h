@@ -100,6 +100,15 @@ int oracle_multi_mol(const oracle_mol *mols, const int32_t *conf_mol, const int6
                      int lbfgs_m, double grad_scale, double grad_tol, double *e_out,
                      double *gnorm_out, int32_t *per_conf_iters, int32_t *conv, int n_threads);
 
+/* The same engine with flags; ORACLE_MM_BATCH gives mmff_optimize_metal_batch
+ * (mlxmolkit/mmff_metal_optimizer.py:77-202): no convergence exit, no polish pass, gnorm_out =
+ * ||g_sc||_2 of the last iterate. */
+#define ORACLE_MM_BATCH 1
+int oracle_multi_mol_ex(const oracle_mol *mols, const int32_t *conf_mol, const int64_t *conf_off,
+                        int n_confs, double *x, int max_iters, int eval_freq, double max_step_per_atom,
+                        int lbfgs_m, double grad_scale, double grad_tol, double *e_out,
+                        double *gnorm_out, int32_t *per_conf_iters, int32_t *conv, int n_threads, int flags);
+
 #ifdef __cplusplus
 }
 #endif

@@ -78,6 +78,9 @@ def lib():
         L.oracle_multi_mol.restype = C.c_int
         L.oracle_multi_mol.argtypes = [vp, vp, vp, C.c_int, vp, C.c_int, C.c_int, C.c_double, C.c_int,
                                        C.c_double, C.c_double, vp, vp, vp, vp, C.c_int]
+        L.oracle_multi_mol_ex.restype = C.c_int
+        L.oracle_multi_mol_ex.argtypes = [vp, vp, vp, C.c_int, vp, C.c_int, C.c_int, C.c_double, C.c_int,
+                                          C.c_double, C.c_double, vp, vp, vp, vp, C.c_int, C.c_int]
         _LIB = L
     return _LIB
 
@@ -218,16 +221,18 @@ def bb_fused(batch: OracleBatch, x, max_iters=500, max_step=0.3, grad_lr=0.1, n_
 
 
 def multi_mol(batch: OracleBatch, x, max_iters=200, eval_freq=10, max_step_per_atom=0.3, lbfgs_m=10,
-              grad_scale=0.1, grad_tol=1e-4, n_threads=None):
-    """Lock-step host L-BFGS. Returns (x_best, E, grad_norms, per_conf_iters, converged, iters_used)."""
+              grad_scale=0.1, grad_tol=1e-4, n_threads=None, batch_mode=False):
+    """Lock-step host L-BFGS. Returns (x_best, E, grad_norms, per_conf_iters, converged, iters_used).
+    ``batch_mode``: the mmff_optimize_metal_batch form (no convergence exit, no polish pass,
+    grad_norms = ||g_sc||_2 of the last iterate)."""
     xx = np.array(x, dtype=np.float64, copy=True).reshape(-1)
     e = np.zeros(batch.n_confs)
     gn = np.zeros(batch.n_confs)
     pci = np.zeros(batch.n_confs, dtype=np.int32)
     conv = np.zeros(batch.n_confs, dtype=np.int32)
-    used = lib().oracle_multi_mol(batch.mols, batch._p(batch.conf_mol), batch._p(batch.conf_off), batch.n_confs,
-                                  xx.ctypes.data, int(max_iters), int(eval_freq), float(max_step_per_atom),
-                                  int(lbfgs_m), float(grad_scale), float(grad_tol), e.ctypes.data,
-                                  gn.ctypes.data, pci.ctypes.data, conv.ctypes.data,
-                                  n_threads or default_threads())
+    used = lib().oracle_multi_mol_ex(batch.mols, batch._p(batch.conf_mol), batch._p(batch.conf_off), batch.n_confs,
+                                     xx.ctypes.data, int(max_iters), int(eval_freq), float(max_step_per_atom),
+                                     int(lbfgs_m), float(grad_scale), float(grad_tol), e.ctypes.data,
+                                     gn.ctypes.data, pci.ctypes.data, conv.ctypes.data,
+                                     n_threads or default_threads(), 1 if batch_mode else 0)
     return xx, e, gn, pci, conv.astype(bool), int(used)

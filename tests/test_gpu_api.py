@@ -98,6 +98,29 @@ def test_fused_multi_mol_result_layout():
         assert r.energies.dtype == np.float32 and np.all(np.isfinite(r.energies))
 
 
+def test_metal_batch_contract():
+    """mmff_optimize_metal_batch (mmff_metal_optimizer.py:77-202): n_iters = max_iters, no
+    per-conformer iterations or flags, grad_norms = ||g_sc||_2 (at least the largest per-atom
+    norm), best-seen positions written back in place, energies at those positions."""
+    from mlxmolkit_archive_amd import mmff_optimize_metal_batch
+    mols = _mols(["CC(=O)OC"], n_confs=3)
+    r = mmff_optimize_metal_batch(mols[0], max_iters=150)
+    assert r.n_iters == 150 and r.n_conformers == 3
+    assert r.per_conf_iters is None and r.converged is None
+    assert r.positions.shape == (3, mols[0].GetNumAtoms(), 3) and r.energies.dtype == np.float32
+    p = oracle_params(mols[0])
+    for k, c in enumerate(mols[0].GetConformers()):
+        assert np.allclose(c.GetPositions(), r.positions[k], atol=1e-6)
+        e_ref = oracle.energy_grad(p, r.positions[k].reshape(-1).astype(np.float64), family=oracle.FAMILY_MEGA)[0]
+        assert abs(r.energies[k] - e_ref) < 1e-3 + 1e-5 * abs(e_ref)
+    assert np.all(r.grad_norms > 0) and np.all(np.isfinite(r.grad_norms))
+
+
+def oracle_params(mol):
+    from mlxmolkit_archive_amd.mmff_params import extract_mmff_params
+    return extract_mmff_params(mol)
+
+
 def test_minimize_nk_initial_energy_and_shapes():
     """tests/test_mmff_minimize_nk.py:25-58 restated: max_iters=0 energy equals the
     independent (oracle) energy; aspirin stays finite and < 50 kcal/mol; shapes kept."""

@@ -8,6 +8,8 @@ Tolerances (stated per check):
   minimization from the same start for >= 90% (BFGS) / >= 80% (L-BFGS) of conformers,
   median within 2e-3; the rest fall into a different local minimum after the FP32 and
   FP64 trajectories part (flexible molecules, hundreds of iterations)
+* multi_mol: iterations used within 20 of the oracle's, converged flags agreeing for >= 90%
+  of conformers, median |dE| < 0.01 kcal/mol where both converged
 * reported FP64 energies equal the FP64 re-evaluation of the returned geometry to 1e-3
 """
 from __future__ import annotations
@@ -261,14 +263,44 @@ def test_multimol_matches_oracle(small_wl):
         if tol == 0.0:
             assert not conv.any() and (pci == max_iters).all() and int(r["iters_used"].cpu()[0]) == max_iters
             continue
-        assert abs(int(r["iters_used"].cpu()[0]) - usedo) <= 0.2 * usedo + 20
-        assert np.mean(conv == convo) >= 0.8
-        both = conv & convo
-        assert both.sum() >= 0.5 * len(conv)
+        used = int(r["iters_used"].cpu()[0])
         e = r["energy"].cpu().numpy()
-        assert np.median(np.abs(e[both] - eo[both])) < 0.1
+        both = conv & convo
+        print(f"\nmulti_mol: iters_used gpu {used} oracle {usedo}; converged gpu {conv.sum()} oracle {convo.sum()} "
+              f"agree {np.mean(conv == convo):.3f}; median |dE| (both converged) {np.median(np.abs(e[both] - eo[both])):.2e}")
+        assert abs(used - usedo) <= 20
+        assert np.mean(conv == convo) >= 0.9
+        assert both.sum() >= 0.8 * len(conv)
+        assert np.median(np.abs(e[both] - eo[both])) < 0.01
         # converged flag agrees with the reported grad norm (reference contract)
         assert np.array_equal(conv, r["grad_norm"].cpu().numpy() <= tol) or tol == 0.0
+
+
+def test_metal_batch_form_matches_oracle(small_wl):
+    """The mmff_optimize_metal_batch form (no convergence exit, no polish pass, L2 norm of the
+    last scaled gradient) against the oracle's batch form: after 20 iterations the geometries,
+    energies and norms agree; after 200 the best-seen energies stay close."""
+    eng = _eng()
+    wl = small_wl
+    db = eng.device_batch(wl.params, wl.conf_counts)
+    xd = torch.from_numpy(wl.x).cuda()
+    ob = oracle.OracleBatch(wl.params, wl.conf_counts)
+    for iters in (20, 200):
+        r = eng.multimol(db, xd, max_iters=iters, grad_tol=0.0, batch_mode=True)
+        torch.cuda.synchronize()
+        xo, eo, gno, pcio, convo, usedo = oracle.multi_mol(ob, wl.x.astype(np.float64), max_iters=iters,
+                                                           grad_tol=0.0, batch_mode=True)
+        x, e, gn = r["x"].cpu().numpy(), r["energy"].cpu().numpy(), r["grad_norm"].cpu().numpy()
+        assert int(r["iters_used"].cpu()[0]) == iters == usedo
+        assert not r["converged"].cpu().numpy().any() and (r["per_conf_iters"].cpu().numpy() == iters).all()
+        print(f"\nbatch form {iters} it: max |dx| {np.abs(x - xo).max():.2e}, max |dE| {np.abs(e - eo).max():.2e}, "
+              f"max rel |d gnorm| {np.max(np.abs(gn - gno) / gno):.2e}")
+        if iters == 20:
+            assert np.abs(x - xo).max() < 1e-3
+            assert np.abs(e - eo).max() < 1e-3 + 1e-5 * np.abs(eo).max()
+            assert np.allclose(gn, gno, rtol=1e-2, atol=1e-5)
+        else:
+            assert np.median(np.abs(e - eo)) < 0.01
 
 
 def test_empty_batch_is_a_no_op():

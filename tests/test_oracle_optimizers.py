@@ -177,3 +177,78 @@ def test_bfgs_h0_scaled_start_converges_to_a_minimum():
     assert sb[0] == 0
     e, g = oracle.energy_grad(p, xb)
     assert np.max(np.abs(g)) < 0.1
+
+
+def _metal_batch_numpy(p, xs, max_iters, eval_freq=10, max_step=0.3, m_hist=10, gsc=0.1):
+    """Pure-numpy loop of mmff_optimize_metal_batch (mmff_metal_optimizer.py:77-202) over the
+    oracle's FP64 MEGA energy: list-based history, per-atom step cap, best-x guard at every
+    sync point, no convergence exit, no polish pass, grad_norms = ||g_sc||_2 of the last
+    iterate (:186). Small inputs only."""
+    out_x, out_e, out_g = [], [], []
+    for x in xs:
+        x = x.copy()
+        e0, g = oracle.energy_grad(p, x, family=oracle.FAMILY_MEGA)
+        g = g * gsc
+        best_e, best_x = e0, x.copy()
+        S, Y, R = [], [], []
+        for it in range(max_iters):
+            q = g.copy()
+            al = []
+            for i in range(len(S) - 1, -1, -1):
+                a = R[i] * np.dot(S[i], q)
+                q = q - a * Y[i]
+                al.append(a)
+            gam = np.dot(S[-1], Y[-1]) / (np.dot(Y[-1], Y[-1]) + 1e-30) if S else 1.0
+            r = gam * q
+            for i in range(len(S)):
+                b = R[i] * np.dot(Y[i], r)
+                r = r + S[i] * (al[len(S) - 1 - i] - b)
+            d = (-r).reshape(-1, 3)
+            disp = np.sqrt((d * d).sum(1, keepdims=True) + 1e-30)
+            d = (d * np.minimum(max_step / disp, 1.0)).reshape(-1)
+            xn = x + d
+            en, gn = oracle.energy_grad(p, xn, family=oracle.FAMILY_MEGA)
+            gn = gn * gsc
+            s, y = xn - x, gn - g
+            sy = np.dot(s, y)
+            if len(S) >= m_hist:
+                S.pop(0), Y.pop(0), R.pop(0)
+            S.append(s), Y.append(y), R.append(1.0 / (sy + 1e-30) if sy > 1e-10 else 0.0)
+            x, g = xn, gn
+            if (it + 1) % eval_freq == 0 and en < best_e:
+                best_e, best_x = en, x.copy()
+        out_x.append(best_x)
+        out_e.append(oracle.energy_grad(p, best_x, family=oracle.FAMILY_MEGA)[0])
+        out_g.append(np.sqrt(np.dot(g, g)))
+    return np.concatenate(out_x), np.array(out_e), np.array(out_g)
+
+
+def test_metal_batch_form_matches_numpy_restatement():
+    """The oracle's mmff_optimize_metal_batch form against an independent numpy restatement of
+    the reference loop on two conformers of a small molecule (two lengths, one not a multiple
+    of eval_freq). The two sum their dot products in different orders, and L-BFGS amplifies
+    that 1e-16 difference by about 10x per ten iterations, so runs stay short."""
+    m, p, x = mol_params("CC(=O)OC", seed=9, k=2)
+    ob = oracle.OracleBatch([p], [2])
+    x64 = x.astype(np.float64)
+    n3 = 3 * p.n_atoms
+    for iters in (17, 30):
+        xo, e, gn, pci, conv, used = oracle.multi_mol(ob, x64, max_iters=iters, grad_tol=1e-4, batch_mode=True)
+        xr, er, gr = _metal_batch_numpy(p, [x64[:n3], x64[n3:]], iters)
+        np.testing.assert_allclose(xo, xr, rtol=0, atol=1e-9)
+        np.testing.assert_allclose(e, er, rtol=1e-12, atol=1e-9)
+        np.testing.assert_allclose(gn, gr, rtol=1e-9, atol=1e-12)
+        assert used == iters
+
+
+def test_metal_batch_form_has_no_polish_and_l2_norm():
+    """With the same main loop, the multi_mol form differs from the batch form only by its
+    polish pass and its max-per-atom norm: at max_iters < 4 (no polish) and grad_tol < 0 (no
+    exit) the geometries agree, and the L2 norm bounds the max per-atom norm from above."""
+    m, p, x = mol_params("CCOC", seed=4, k=2)
+    ob = oracle.OracleBatch([p], [2])
+    x64 = x.astype(np.float64)
+    a = oracle.multi_mol(ob, x64, max_iters=3, eval_freq=1, grad_tol=-1.0)
+    b = oracle.multi_mol(ob, x64, max_iters=3, eval_freq=1, grad_tol=-1.0, batch_mode=True)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert (b[2] >= a[2]).all() and (b[2] <= np.sqrt(p.n_atoms) * a[2] + 1e-12).all()
