@@ -1,15 +1,18 @@
 #!/usr/bin/env python3
 """Benchmark: converged conformers/s of batched MMFF94 minimization on one MI355X.
 
-Workload (default): 1,000 seeded molecules x 10 conformers (about 30-90 atoms with H),
-dense BFGS of the run_mmff path (reference mmff_minimize_nk), maxIters 2000, grad tol 1e-4.
-Inputs are uploaded to HBM before timing; one step = one full minimization of every
-conformer. A conformer counts as converged when its status is 0 AND the FP64 energy of
-its final geometry is within 0.1 kcal/mol of the FP64 oracle minimization from the same
-start; the oracle runs on a seeded sample (the cpu_baseline leg), and the within-0.1
-fraction measured there scales the GPU's status-0 count.
+Workload (default): the ragged batch, 10,000 seeded molecules x 20 conformers (heavy atoms
+log-uniform in 8-60, 12-157 atoms with H, up to 3 fused or spiro rings), dense BFGS of the
+run_mmff path (reference mmff_minimize_nk), maxIters 2000, grad tol 1e-4. Inputs are
+uploaded to HBM before timing; one step = one full minimization of every conformer (one
+launch per size bucket, on side streams). A conformer counts as converged when its status is
+0 AND the FP64 energy of its final geometry is within 0.1 kcal/mol of the FP64 oracle
+minimization from the same start; the oracle runs on a seeded sample (the cpu_baseline leg),
+and the within-0.1 fraction measured there scales the GPU's status-0 count.
 
-Prints one JSON line (see the keys below). --gpus N > 1 exits: the package runs on one GPU.
+``--workload medium`` (1,000 x 10) and ``--workload chain --lbfgs`` (2,000 x 8, L-BFGS m = 10)
+give the other configurations' lines. Prints one JSON line (see the keys below). --gpus N > 1
+exits: the package runs on one GPU.
 """
 from __future__ import annotations
 
@@ -26,20 +29,43 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK = 8.0e12       # B/s, MI355X
+FP32_PEAK = 157.3e12    # FLOP/s, MI355X dense FP32 vector (counts packed FP32 FMA issue)
+
+# FP32 operations per term evaluation of the TG family (the nk kernels' formulas in
+# csrc/mmff_terms.h), counted on the source: add/sub/mul/rcp/rsqrt/sqrt/acos/cos = 1,
+# FMA = 2. (energy only, energy + gradient); pairs are the merged vdW + electrostatic pair.
+TERM_FLOPS = {"bond": (19, 31), "angle": (34, 79), "strbend": (40, 95), "oop": (45, 110),
+              "torsion": (50, 140), "pair": (32, 50)}
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(wl, counters, dense=True, m_hist=8):
-    """B = sum_c iters*(12n^2+40n) + grads*(T+8n) + evals*(T+4n) (L-BFGS: 16mn+48n per iteration)."""
+def _mol_counts(hb):
+    d = hb.mol_desc.reshape(-1, 16).astype(np.float64)
+    return {"bond": d[:, 2], "angle": d[:, 4], "strbend": d[:, 6], "oop": d[:, 8], "torsion": d[:, 10],
+            "pair": d[:, 12]}
+
+
+def algorithmic_cost(wl, hb, counters, dense=True, m_hist=8):
+    """Per-conformer algorithmic (bytes, FP32 flops) of one minimization from its counters.
+
+    bytes: iters*(12n^2+40n) + grads*(T+8n) + evals*(T+4n), T = canonical table bytes
+    (L-BFGS: 16mn+48n per iteration); flops: iters*(10n^2+30n) (L-BFGS 8mn+30n) +
+    grads*F_eg + evals*F_e with the per-term counts above."""
     from mlxmolkit_archive_amd.packing import canonical_term_bytes
     tb = np.repeat([canonical_term_bytes(p) for p in wl.params], wl.conf_counts).astype(np.float64)
     n = 3.0 * wl.atoms_per_conf()
     it, gr, ev = (counters[:, k].astype(np.float64) for k in range(3))
     per_it = 12 * n * n + 40 * n if dense else 16 * m_hist * n + 48 * n
-    return float(np.sum(it * per_it + gr * (tb + 8 * n) + ev * (tb + 4 * n)))
+    B = it * per_it + gr * (tb + 8 * n) + ev * (tb + 4 * n)
+    cnt = _mol_counts(hb)
+    fe = sum(cnt[k] * TERM_FLOPS[k][0] for k in TERM_FLOPS)[hb.conf_mol]
+    feg = sum(cnt[k] * TERM_FLOPS[k][1] for k in TERM_FLOPS)[hb.conf_mol]
+    f_it = 10 * n * n + 30 * n if dense else 8 * m_hist * n + 30 * n
+    F = it * f_it + gr * feg + ev * fe
+    return B, F
 
 
 def main():
@@ -47,7 +73,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="medium", choices=["small", "medium", "ragged", "chain"])
+    ap.add_argument("--workload", default="ragged", choices=["small", "medium", "ragged", "chain"],
+                    help="ragged (default): 10,000 molecules x 20 conformers, 10-150 atoms, the configuration "
+                         "the headline target is quoted on; medium: 1,000 x 10; chain: 2,000 x 8 (with --lbfgs)")
     ap.add_argument("--mols", type=int, default=None)
     ap.add_argument("--confs", type=int, default=None)
     ap.add_argument("--max-iters", type=int, default=2000)
@@ -55,10 +83,11 @@ def main():
     ap.add_argument("--lbfgs", action="store_true")
     ap.add_argument("--lbfgs-m", type=int, default=10,
                     help="L-BFGS history length with --lbfgs (the chain configuration uses m = 10)")
-    ap.add_argument("--sample", type=int, default=4096, help="conformers checked by the FP64 oracle")
+    ap.add_argument("--sample", type=int, default=2048, help="conformers checked by the FP64 oracle")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-graph", action="store_true", help="launch eagerly instead of replaying a HIP graph")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay the step from a HIP graph (then only whole-step times are measured)")
     ap.add_argument("--engine", default="nk", choices=["nk", "multimol"],
                     help="nk: in-kernel BFGS/L-BFGS of the run_mmff path (headline); multimol: the lock-step "
                          "L-BFGS of mmff_optimize_metal_multi_mol run on the device")
@@ -116,12 +145,12 @@ def main():
         runner.run(x_dev, x_out)
         torch.cuda.synchronize()
         log(f"warmup {w + 1}/{args.warmup} done")
-    graph = None
-    if not args.no_graph:
-        graph = runner.capture(x_dev, x_out)
-        log("step captured in a HIP graph (queue resets + one minimize launch per bucket)")
+    graph = runner.capture(x_dev, x_out) if args.graph else None
+    nl = len(runner.launches)
     ev_s = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
     ev_e = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ev_l = [[(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(nl)]
+            for _ in range(args.steps)]
     torch.cuda.synchronize()
     t_start = time.perf_counter()
     for k in range(args.steps):
@@ -129,21 +158,41 @@ def main():
         if graph is not None:
             graph.replay()
         else:
-            runner.run(x_dev, x_out)
+            runner.run(x_dev, x_out, events=ev_l[k])
         ev_e[k].record(stream)
     torch.cuda.synchronize()
     res = engine.NkResult(x_out, runner.e, runner.st, runner.cnt)
     wall = (time.perf_counter() - t_start) / args.steps
     kern_ms = float(np.mean([s.elapsed_time(e) for s, e in zip(ev_s, ev_e)]))
-    log(f"timed {args.steps} steps: {wall * 1e3:.1f} ms/step wall, {kern_ms:.1f} ms/step in kernels")
+    launch_ms = None
+    if graph is None:
+        launch_ms = [float(np.mean([ev_l[k][j][0].elapsed_time(ev_l[k][j][1]) for k in range(args.steps)]))
+                     for j in range(nl)]
+    log(f"timed {args.steps} steps: {wall * 1e3:.1f} ms/step wall, {kern_ms:.1f} ms/step on the GPU"
+        + ("" if launch_ms is None else "; per launch " + ", ".join(f"{t:.1f}" for t in launch_ms) + " ms"))
 
     st = res.status.cpu().numpy()
     cnt = res.counters.cpu().numpy()
     e_gpu = res.energy.cpu().numpy()
     xg = res.x.cpu().numpy()
     n_status0 = int((st == 0).sum())
-    B = algorithmic_bytes(wl, cnt, dense=not args.lbfgs, m_hist=args.lbfgs_m)
-    achieved = B / (kern_ms * 1e-3)
+    Bc, Fc = algorithmic_cost(wl, db.host, cnt, dense=not args.lbfgs, m_hist=args.lbfgs_m)
+    # the roofline is priced on the dominant launch: its conformers' algorithmic bytes and
+    # flops over its own measured duration (the launches of a step run concurrently)
+    launches = []
+    for j, ln in enumerate(runner.launches):
+        sel = ln["order"].cpu().numpy()
+        launches.append({"conformers": int(ln["n"]), "max_n": int(ln["max_n"]), "block": int(ln["opts"].block),
+                         "hessian": {1: "lds", 2: "global"}.get(int(ln["opts"].hess_place), "auto"),
+                         "ms": None if launch_ms is None else round(launch_ms[j], 3),
+                         "bytes": float(Bc[sel].sum()), "flops": float(Fc[sel].sum())})
+    if launch_ms is not None:
+        dom = int(np.argmax(launch_ms))
+        t_dom = launch_ms[dom] * 1e-3
+    else:
+        dom, t_dom = int(np.argmax([l["bytes"] for l in launches])), kern_ms * 1e-3
+    achieved = launches[dom]["bytes"] / t_dom
+    flops_rate = launches[dom]["flops"] / t_dom
 
     # ---- cpu_baseline leg: FP64 oracle on a seeded sample (also the Delta-E checker) ----
     cpu = None
@@ -174,29 +223,43 @@ def main():
                                                        h0=h0s)
         t_cpu = time.perf_counter() - t0
         e_gpu_fp64, _, _ = oracle.energy_grad_batch(ob, xgs, family=oracle.FAMILY_TG, n_threads=threads)
-        dE = np.abs(e_gpu_fp64 - e_ref)
+        dEs = e_gpu_fp64 - e_ref
+        dE = np.abs(dEs)
         ok0 = st[pick] == 0
         within = ok0 & (dE <= 0.1)
         frac = float(within.sum() / max(ok0.sum(), 1))
+        both = ok0 & (st_ref == 0)
         cpu = {"value": round(float((st_ref == 0).sum() / t_cpu), 3), "unit": "converged conformers/s",
                "cores": int(threads), "kind": "port",
                "sample": f"{S} seeded conformers of the same workload, FP64 oracle "
                          f"{'L-BFGS' if args.lbfgs else 'BFGS'} (same control flow), {t_cpu:.1f}s wall"}
-        sample_info = {"sample": S, "status0_in_sample": int(ok0.sum()), "within_0.1_in_sample": int(within.sum()),
+        sample_info = {"sample": S, "status0_in_sample": int(ok0.sum()), "oracle_status0": int((st_ref == 0).sum()),
+                       "within_0.1_in_sample": int(within.sum()),
+                       # misses among conformers both sides converged: did the GPU end lower or higher?
+                       "miss_gpu_lower": int((both & (dEs < -0.1)).sum()),
+                       "miss_gpu_higher": int((both & (dEs > 0.1)).sum()),
                        "median_dE": float(np.median(dE)), "p95_dE": float(np.percentile(dE, 95)),
+                       "mean_iters_gpu_vs_oracle": [round(float(cnt[pick][both, 0].mean()), 1) if both.any() else None,
+                                                    round(float(cnt_ref[both, 0].mean()), 1) if both.any() else None],
                        "max_reported_vs_fp64_dE": float(np.max(np.abs(e_gpu[pick] - e_gpu_fp64)))}
         log(f"cpu_baseline {t_cpu:.1f}s; within 0.1 kcal/mol: {within.sum()}/{ok0.sum()} status-0 sampled "
-            f"(median |dE| {np.median(dE):.2e})")
-    # measured HBM bytes per step for this workload: rocprofv3 FETCH_SIZE / WRITE_SIZE passes
-    # scaled by the copy-kernel calibration (profiles/k_minimize_nk_medium-1000x10.json)
-    traffic = None
-    prof = ROOT / "profiles" / "k_minimize_nk_medium-1000x10.json"
-    if args.workload == "medium" and args.mols is None and args.confs is None and not args.lbfgs and prof.exists():
-        traffic = float(json.loads(prof.read_text())["traffic_bytes_per_step"])
+            f"(median |dE| {np.median(dE):.2e}; misses lower/higher {sample_info['miss_gpu_lower']}/"
+            f"{sample_info['miss_gpu_higher']})")
+    # measured HBM bytes per launch of the dominant kernel on this workload: rocprofv3
+    # FETCH_SIZE / WRITE_SIZE passes over the same command, scaled by the copy-kernel
+    # calibration, read from the committed profile summary (a separate profiler run)
+    traffic, traffic_src = None, None
+    default_shape = args.mols is None and args.confs is None and not args.lbfgs and pc is None
+    prof = ROOT / "profiles" / f"k_minimize_nk_{args.workload}_traffic.json"
+    if default_shape and prof.exists():
+        pj = json.loads(prof.read_text())
+        traffic = float(pj["dominant_launch_bytes"])
+        traffic_src = f"profiles/{prof.name} (rocprofv3 --pmc, separate run of this command)"
     conv_est = n_status0 * (frac if frac is not None else 1.0)
     value = conv_est / (kern_ms * 1e-3)
     ok = st == 0
     mean_iters = float(cnt[ok, 0].mean()) if ok.any() else float("nan")
+    dl = launches[dom]
     out = {
         "metric": "converged conformers/s at |dE|<=0.1 kcal/mol vs FP64; mean iters to converge",
         "value": round(value, 2),
@@ -208,7 +271,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 terms, fp64 energy sums",
+        "dtype": "f32 terms, f64 energy sums",
         "data": "synthetic: seeded molecules, typed by the built-in MMFF94-style typer, embedded + N(0,0.1A) noise",
         "config": {"workload": f"{args.workload}: {len(wl.params)} molecules x {wl.conf_counts[0]} conformers",
                    "algorithm": (f"L-BFGS m={args.lbfgs_m}" if args.lbfgs else "dense BFGS")
@@ -216,27 +279,31 @@ def main():
                    "grad_tol": args.grad_tol, "atoms_min_max": [int(wl.atoms_per_conf().min()),
                                                                 int(wl.atoms_per_conf().max())],
                    "parallelism": "1 GPU, one workgroup per conformer, persistent size-bucketed launches",
-                   "launch": "eager" if graph is None else "HIP graph replay"},
+                   "launch": "eager, one launch per size bucket on side streams" if graph is None
+                             else "HIP graph replay"},
         "mean_iters_to_converge": round(mean_iters, 1),
         "status0_fraction": round(n_status0 / wl.n_confs, 4),
         "within_0.1_fraction_sample": None if frac is None else round(frac, 4),
         "wall_ms_per_step": round(wall * 1e3, 3),
+        "host_s": {"generate_type_embed": round(t_gen, 1), "pack_upload_plan": round(t_pack, 2)},
+        "launches": launches,
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK, 4),
                      "traffic": None if traffic is None else round(traffic),
-                     "traffic_frac": None if traffic is None else round(traffic / (kern_ms * 1e-3) / HBM_PEAK, 4),
-                     "note": (("achieved = algorithmic bytes (16mn+48n per L-BFGS iteration + "
-                               "table/vector bytes per evaluation) of one step / the step's GPU time (HIP events "
-                               "around its k_minimize_nk launches); the (s, y) history lives in LDS up to n = 192, "
-                               "in global memory (L2 / Infinity Cache resident) above") if args.lbfgs else
-                              ("achieved = algorithmic bytes (12n^2+40n per BFGS iteration + table/vector bytes "
-                               "per evaluation) of one step / the step's GPU time (HIP events around its "
-                               "k_minimize_nk launches); it exceeds the HBM peak because each conformer's "
-                               "Hessian stays on chip: in LDS up to n = 96, above that in L2 and the 256 MB "
-                               "Infinity Cache (one slot per resident workgroup). traffic = measured bytes per "
-                               "step between L2 and memory (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0, copy-"
-                               "calibrated; Infinity Cache hits count as traffic); traffic_frac = traffic / "
-                               "step time / HBM peak"))},
+                     "traffic_frac": None if traffic is None else round(traffic / t_dom / HBM_PEAK, 4),
+                     "traffic_source": traffic_src,
+                     "kernel": f"k_minimize_nk, {dl['block']} threads, Hessian in {dl['hessian']} memory, "
+                               f"{dl['conformers']} conformers n<={dl['max_n']}",
+                     "note": ("achieved = algorithmic bytes of the dominant launch's conformers (per BFGS "
+                              "iteration 12n^2+40n, per L-BFGS iteration 16mn+48n, plus table and vector bytes "
+                              "per evaluation, from the kernel's own counters) / that launch's duration "
+                              "(HIP events on its stream). Above 1: the Hessians stay on chip (LDS, L2, "
+                              "Infinity Cache), so the kernel moves the algorithm's bytes faster than DRAM. "
+                              "traffic = measured bytes between L2 and the memory fabric for that launch.")},
+        "roofline_fp32": {"bound": "valu", "achieved": round(flops_rate / 1e12, 2), "peak": FP32_PEAK / 1e12,
+                          "unit": "TFLOP/s", "frac": round(flops_rate / FP32_PEAK, 4),
+                          "note": "algorithmic FP32 flops of the same launch (10n^2+30n per BFGS iteration, "
+                                  "per-term counts of the TG formulas per evaluation) / its duration"},
         "cpu_baseline": cpu,
         "precond": pc_info,
         "check": sample_info,

@@ -60,7 +60,8 @@ enum mmffhip_hess_place {
  *   sb_idx   u32x2 {i|j<<16, k}        sb_p    f32x4 {kba_ijk, kba_kji, r0_ij, r0_kj}, sb_t0 f32 (deg)
  *   oop_idx  u32x2 {i|j<<16, k|l<<16}  oop_k   f32
  *   tor_idx  u32x2 {i|j<<16, k|l<<16}  tor_p   f32x4 {V1, V2, V3, 0}
- *   pair_ij  u32  i | j<<16            pair_p  f32x4 {R*, eps, q_i q_j * (0.75 if 1-4 else 1), 0}
+ *   pair_ij  u32  i | j<<16            pair_p  f32x4 {R*, eps, q_i q_j * (0.75 if 1-4 else 1),
+ *                                                    1 / max(R*, 1e-8)}
  * (vdW and electrostatic pairs of the reference are merged into one pair list.) */
 typedef struct {
   const int32_t *mol_desc;

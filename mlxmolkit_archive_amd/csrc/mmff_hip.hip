@@ -109,50 +109,64 @@ __device__ __forceinline__ MolDesc load_desc(const int32_t* md) {
 }
 
 // ------------------------------------------------------------------------------------
-// 64-bit fixed-point gradient accumulation in LDS (scale 2^32). The split through FP64
-// is exact for |v| < 2^31, so the accumulated sum equals the exact sum of the FP32
-// contributions truncated to 2^-32, independent of arrival order.
+// 64-bit fixed-point gradient accumulation in LDS (scale 2^28). Each FP32 contribution is
+// rounded once to a multiple of 2^-28, so the accumulated sum is the exact sum of the
+// rounded contributions, independent of arrival order.
+//
+// Conversion: for |v| < 2^23 the double FX_MAGIC + v, FX_MAGIC = 1.5 * 2^24, lies in
+// [2^24, 2^25), whose spacing is 2^-28, so the addition rounds v to the nearest multiple of
+// 2^-28 (ties to even) and the low bits of the sum hold that multiple: bits(FX_MAGIC + v) -
+// bits(FX_MAGIC) = round(v * 2^28) as a two's-complement integer. Round-to-nearest-even is
+// symmetric, so to_fx(-v) == -to_fx(v) exactly. One conversion, one FP64 add and one 64-bit
+// subtract, against the sign/split/recombine sequence of a truncating conversion.
+constexpr double FX_MAGIC = 25165824.0;   // 1.5 * 2^24
+constexpr float FX_CAP = 8388000.0f;      // |v| is capped just below 2^23
+constexpr double FX_UNIT = 3.7252902984619140625e-09;  // 2^-28
+
 __device__ __forceinline__ unsigned long long to_fx(float v) {
-  // v * 2^32 rounded toward zero, as a two's-complement 64-bit integer; so that
-  // to_fx(-v) == -to_fx(v) exactly. The magnitude is split into its integer part and the
-  // fraction |v| - trunc|v|, which FP32 computes exactly (Sterbenz for |v| >= 1);
-  // NaN maps to 0 and magnitudes are capped just below 2^31.
-  const float a = (v == v) ? fminf(fabsf(v), 2147483520.0f) : 0.0f;
-  const float h = truncf(a);
-  const unsigned int lo = (unsigned int)((a - h) * 4294967296.0f);
-  const unsigned long long m = ((unsigned long long)(unsigned int)h << 32) | (unsigned long long)lo;
-  return v < 0.0f ? (unsigned long long)(-(long long)m) : m;
+  // NaN maps to 0
+  const float c = (v == v) ? fminf(fmaxf(v, -FX_CAP), FX_CAP) : 0.0f;
+  const double y = (double)c + FX_MAGIC;
+  return (unsigned long long)(__double_as_longlong(y) - __double_as_longlong(FX_MAGIC));
 }
 
 __device__ __forceinline__ float from_fx(unsigned long long a) {
-  int hi = (int)(a >> 32);
-  unsigned int lo = (unsigned int)(a & 0xffffffffull);
-  return (float)((double)hi + (double)lo * 2.3283064365386962890625e-10);
+  return (float)((double)(long long)a * FX_UNIT);
 }
 
+// The accumulators of a conformer with N atoms are three planes of N words (x, y, z), so
+// lanes updating consecutive atoms touch consecutive 8-byte words (no bank conflicts); the
+// gradient of coordinate i = 3a + c is plane c, word a.
 template <bool G>
 struct LdsAcc {
   unsigned long long* g;
+  int N;
   __device__ __forceinline__ void add(int a, float x, float y, float z) {
     if (G) {
-      atomicAdd(&g[3 * a], to_fx(x));
-      atomicAdd(&g[3 * a + 1], to_fx(y));
-      atomicAdd(&g[3 * a + 2], to_fx(z));
+      atomicAdd(&g[a], to_fx(x));
+      atomicAdd(&g[N + a], to_fx(y));
+      atomicAdd(&g[2 * N + a], to_fx(z));
     }
   }
   // +g on atom a, -g on atom b: one conversion serves both (to_fx(-x) == -to_fx(x))
   __device__ __forceinline__ void add_pair(int a, int b, float x, float y, float z) {
     if (G) {
       const unsigned long long fx = to_fx(x), fy = to_fx(y), fz = to_fx(z);
-      atomicAdd(&g[3 * a], fx);
-      atomicAdd(&g[3 * a + 1], fy);
-      atomicAdd(&g[3 * a + 2], fz);
-      atomicSub(&g[3 * b], fx);
-      atomicSub(&g[3 * b + 1], fy);
-      atomicSub(&g[3 * b + 2], fz);
+      atomicAdd(&g[a], fx);
+      atomicAdd(&g[N + a], fy);
+      atomicAdd(&g[2 * N + a], fz);
+      atomicSub(&g[b], fx);
+      atomicSub(&g[N + b], fy);
+      atomicSub(&g[2 * N + b], fz);
     }
   }
 };
+
+// gradient of coordinate i (= 3a + c) from the accumulator planes of an N-atom conformer
+__device__ __forceinline__ float gread(const unsigned long long* g, int i, int N) {
+  const int a = i / 3;
+  return from_fx(g[(i - 3 * a) * N + a]);
+}
 
 // ------------------------------------------------------------------------------------
 // energy (+ gradient) of one conformer whose positions are in LDS, spread over NT lanes.
@@ -319,7 +333,7 @@ __device__ __forceinline__ TermData load_term(const TView& V, int t, int c1, int
 template <int FAM, bool G, bool PT, int NT, bool PREF = true>
 __device__ __forceinline__ double eval_terms(const TView& TV, const MolDesc& m, const float* xs,
                                              unsigned long long* gacc, int diel, uint32_t mask, double* et) {
-  LdsAcc<G> acc{gacc};
+  LdsAcc<G> acc{gacc, m.n_atoms};
   double e = 0.0;
   // bonded terms (one generic loop, next term prefetched when PREF)
   const int c1 = m.nb, c2 = c1 + m.na, c3 = c2 + m.nsb, c4 = c3 + m.noop, c5 = c4 + m.ntor;
@@ -385,14 +399,24 @@ __device__ __forceinline__ double eval_terms(const TView& TV, const MolDesc& m, 
       const uint32_t nija = TV.pair_ij[min(un, last)], nijb = TV.pair_ij[min(un + NT, last)];
       const float4 npa = TV.pair_p[min(un, last)], npb = TV.pair_p[min(un + NT, last)];
       float eva, eea;
-      t_pair<FAM, G>(xs, acc, (int)(ija & 0xffff), (int)(ija >> 16), pa.x, ke * pa.y, kq * pa.z, diel, eva, eea);
-      ev_acc += eva;
-      ee_acc += eea;
+      t_pair<FAM, G>(xs, acc, (int)(ija & 0xffff), (int)(ija >> 16), pa.w, ke * pa.y, kq * pa.z, diel, eva, eea);
+      // without per-term energies a pair's two parts are added in FP32 first (one rounding,
+      // as small as the parts' own), then to the lane's FP64 sum
+      if (PT) {
+        ev_acc += eva;
+        ee_acc += eea;
+      } else {
+        ev_acc += (double)(eva + eea);
+      }
       if (u + NT < np) {
         float evb, eeb;
-        t_pair<FAM, G>(xs, acc, (int)(ijb & 0xffff), (int)(ijb >> 16), pb.x, ke * pb.y, kq * pb.z, diel, evb, eeb);
-        ev_acc += evb;
-        ee_acc += eeb;
+        t_pair<FAM, G>(xs, acc, (int)(ijb & 0xffff), (int)(ijb >> 16), pb.w, ke * pb.y, kq * pb.z, diel, evb, eeb);
+        if (PT) {
+          ev_acc += evb;
+          ee_acc += eeb;
+        } else {
+          ev_acc += (double)(evb + eeb);
+        }
       }
       ija = nija;
       ijb = nijb;
@@ -447,18 +471,18 @@ __device__ __forceinline__ float dpp_f(float v) {
 }
 
 // four consecutive elements (t, j..j+3) of a symmetric matrix in padded packed storage
-// (column c at pcol(c), cb = pcol(t), pj = pcol(j)), with the matching {dg, g} entries
+// (column c at pcol(c), cb = pcol(t), pj = pcol(j)), with the matching gradient entries
 __device__ __forceinline__ void hrow_load4(const float* H, const float2* VG, int t, int cb, int j, int pj,
-                                           float (&h)[4], float2 (&w)[4]) {
+                                           float (&h)[4], float (&w)[4]) {
   const int p1 = pj + ((j + 4) & ~3), p2 = p1 + ((j + 5) & ~3), p3 = p2 + ((j + 6) & ~3);
   h[0] = H[j <= t ? cb + j : pj + t];
   h[1] = H[j + 1 <= t ? cb + j + 1 : p1 + t];
   h[2] = H[j + 2 <= t ? cb + j + 2 : p2 + t];
   h[3] = H[j + 3 <= t ? cb + j + 3 : p3 + t];
-  w[0] = VG[j];
-  w[1] = VG[j + 1];
-  w[2] = VG[j + 2];
-  w[3] = VG[j + 3];
+  w[0] = VG[j].y;
+  w[1] = VG[j + 1].y;
+  w[2] = VG[j + 2].y;
+  w[3] = VG[j + 3].y;
 }
 
 // one row of a column quad of the full Hessian gains a pending rank-2 update: column t's
@@ -470,31 +494,25 @@ __device__ __forceinline__ void hquad_upd(float4& h, float2 w, const float (&ux)
   h.w = fmaf(uy[3], w.y, fmaf(ux[3], w.x, h.w));
 }
 
-// accumulate the row's {dg, g} = v into the quad's two dot products
-__device__ __forceinline__ void hquad_dot(const float4& h, float2 v, float (&a1)[4], float (&a2)[4]) {
-  a1[0] = fmaf(h.x, v.x, a1[0]);
-  a2[0] = fmaf(h.x, v.y, a2[0]);
-  a1[1] = fmaf(h.y, v.x, a1[1]);
-  a2[1] = fmaf(h.y, v.y, a2[1]);
-  a1[2] = fmaf(h.z, v.x, a1[2]);
-  a2[2] = fmaf(h.z, v.y, a2[2]);
-  a1[3] = fmaf(h.w, v.x, a1[3]);
-  a2[3] = fmaf(h.w, v.y, a2[3]);
+// accumulate the row's gradient entry g into the quad's dot products
+__device__ __forceinline__ void hquad_dot(const float4& h, float g, float (&a)[4]) {
+  a[0] = fmaf(h.x, g, a[0]);
+  a[1] = fmaf(h.y, g, a[1]);
+  a[2] = fmaf(h.z, g, a[2]);
+  a[3] = fmaf(h.w, g, a[3]);
 }
 
 // Rows [j, j1) of a column quad of the full Hessian (row r at base[r * nq]): accumulate
-// H v for v = {dg, g} into a1, a2, with H the stored rows plus the NU oldest pending rank-2
-// updates (update k: row vectors {s, h} in S[k], column coefficients cx[k], cy[k]). When all
-// NPM slots are pending the updates are added to each element, oldest first, and the rows
-// are written back. Otherwise the stored rows are only read, and the updates' part of H v is
-// left as four dot products per update over these rows in lr (s.dg, h.dg, s.g, h.g), which
-// the caller combines with the column coefficients after summing the quad's lanes. HQ rows
-// are loaded together.
+// H g into a, with H the stored rows plus the NU oldest pending rank-2 updates (update k:
+// row vectors {s, h} in S[k], column coefficients cx[k], cy[k]). When all NPM slots are
+// pending the updates are added to each element, oldest first, and the rows are written
+// back. Otherwise the stored rows are only read, and the updates' part of H g is left as two
+// dot products per update over these rows in lr (s.g, h.g), which the caller combines with
+// the column coefficients after summing the quad's lanes. HQ rows are loaded together.
 template <int HQ, int NU, int NPM>
 __device__ __forceinline__ void hquad_rows(float4* base, int nq, int j, int j1, const float2* VG,
                                            const float2* const (&S)[NPM], const float (&cx)[NPM][4],
-                                           const float (&cy)[NPM][4], float (&a1)[4], float (&a2)[4],
-                                           float (&lr)[4][NPM]) {
+                                           const float (&cy)[NPM][4], float (&a)[4], float (&lr)[2][NPM]) {
   constexpr bool FLUSH = NU == NPM;
   for (; j + HQ - 1 < j1; j += HQ) {
     float4 h[HQ];
@@ -502,20 +520,18 @@ __device__ __forceinline__ void hquad_rows(float4* base, int nq, int j, int j1, 
     for (int u = 0; u < HQ; u++) h[u] = base[(size_t)(j + u) * nq];
 #pragma unroll
     for (int u = 0; u < HQ; u++) {
-      const float2 v = VG[j + u];
+      const float g = VG[j + u].y;
 #pragma unroll
       for (int k = 0; k < NU; k++) {
         const float2 w = S[k][j + u];
         if (FLUSH) {
           hquad_upd(h[u], w, cx[k], cy[k]);
         } else {
-          lr[0][k] = fmaf(w.x, v.x, lr[0][k]);
-          lr[1][k] = fmaf(w.y, v.x, lr[1][k]);
-          lr[2][k] = fmaf(w.x, v.y, lr[2][k]);
-          lr[3][k] = fmaf(w.y, v.y, lr[3][k]);
+          lr[0][k] = fmaf(w.x, g, lr[0][k]);
+          lr[1][k] = fmaf(w.y, g, lr[1][k]);
         }
       }
-      hquad_dot(h[u], v, a1, a2);
+      hquad_dot(h[u], g, a);
     }
     if (FLUSH) {
 #pragma unroll
@@ -524,20 +540,18 @@ __device__ __forceinline__ void hquad_rows(float4* base, int nq, int j, int j1, 
   }
   for (; j < j1; j++) {
     float4 h = base[(size_t)j * nq];
-    const float2 v = VG[j];
+    const float g = VG[j].y;
 #pragma unroll
     for (int k = 0; k < NU; k++) {
       const float2 w = S[k][j];
       if (FLUSH) {
         hquad_upd(h, w, cx[k], cy[k]);
       } else {
-        lr[0][k] = fmaf(w.x, v.x, lr[0][k]);
-        lr[1][k] = fmaf(w.y, v.x, lr[1][k]);
-        lr[2][k] = fmaf(w.x, v.y, lr[2][k]);
-        lr[3][k] = fmaf(w.y, v.y, lr[3][k]);
+        lr[0][k] = fmaf(w.x, g, lr[0][k]);
+        lr[1][k] = fmaf(w.y, g, lr[1][k]);
       }
     }
-    hquad_dot(h, v, a1, a2);
+    hquad_dot(h, g, a);
     if (FLUSH) base[(size_t)j * nq] = h;
   }
 }
@@ -546,15 +560,14 @@ __device__ __forceinline__ void hquad_rows(float4* base, int nq, int j, int j1, 
 template <int HQ, int NPM, int NU = NPM>
 __device__ __forceinline__ void hquad_dispatch(int npend, float4* base, int nq, int j, int j1, const float2* VG,
                                                const float2* const (&S)[NPM], const float (&cx)[NPM][4],
-                                               const float (&cy)[NPM][4], float (&a1)[4], float (&a2)[4],
-                                               float (&lr)[4][NPM]) {
+                                               const float (&cy)[NPM][4], float (&a)[4], float (&lr)[2][NPM]) {
   if constexpr (NU == 0) {
-    hquad_rows<HQ, 0, NPM>(base, nq, j, j1, VG, S, cx, cy, a1, a2, lr);
+    hquad_rows<HQ, 0, NPM>(base, nq, j, j1, VG, S, cx, cy, a, lr);
   } else {
     if (npend == NU)
-      hquad_rows<HQ, NU, NPM>(base, nq, j, j1, VG, S, cx, cy, a1, a2, lr);
+      hquad_rows<HQ, NU, NPM>(base, nq, j, j1, VG, S, cx, cy, a, lr);
     else
-      hquad_dispatch<HQ, NPM, NU - 1>(npend, base, nq, j, j1, VG, S, cx, cy, a1, a2, lr);
+      hquad_dispatch<HQ, NPM, NU - 1>(npend, base, nq, j, j1, VG, S, cx, cy, a, lr);
   }
 }
 
@@ -656,7 +669,7 @@ __global__ void __launch_bounds__(NT) k_energy_grad(KBatch B, const float* __res
   }
   if (threadIdx.x == 0) e_out[c] = e;
   if (g_out)
-    for (int i = threadIdx.x; i < n; i += NT) g_out[off + i] = from_fx(gacc[i]);
+    for (int i = threadIdx.x; i < n; i += NT) g_out[off + i] = gread(gacc, i, m.n_atoms);
 }
 
 // ------------------------------------------------------------------------------------
@@ -706,14 +719,17 @@ __host__ __device__ inline int pcol(int c) {
 __host__ __device__ inline size_t packed_floats(int n) { return (size_t)pcol(n); }
 
 // LDS footprint (bytes) of the nk kernel
-__host__ __device__ inline size_t nk_lds_bytes(int algo, bool hlds, int nt, int max_n, int m, size_t tbytes = 0) {
+// hm: Hessian placement (0 packed LDS, 1 full HBM, 2 packed HBM); only the full-HBM BFGS
+// kernel keeps the extra pending-update slots
+__host__ __device__ inline size_t nk_lds_bytes(int algo, int hm, int nt, int max_n, int m, size_t tbytes = 0) {
+  const bool hlds = hm == 0;
   size_t nw = (size_t)nt / 64;
   size_t b = 16 * nw * sizeof(double) + 64 + 2 * (size_t)LBFGS_MAX_M * sizeof(double);
   b += (size_t)max_n * 8;       // gradient accumulators
   b += (size_t)max_n * 4 * 4;   // xs, d, xo, hd
   b += (size_t)max_n * 16;      // V = {dg, g, pending xi, pending hd}
   b += (size_t)max_n * 8;       // per-lane coefficients of the pending update
-  if (algo == MMFFHIP_BFGS && !hlds) b += (size_t)(HPEND_MAX - 1) * max_n * 16;  // more pending updates
+  if (algo == MMFFHIP_BFGS && hm == 1) b += (size_t)(HPEND_MAX - 1) * max_n * 16;  // more pending updates
   if (algo == MMFFHIP_LBFGS) {
     if (hlds) b += 2 * (size_t)m * max_n * 4;
   } else if (hlds) {
@@ -836,14 +852,14 @@ __global__ void __launch_bounds__(NT, (HM == 1 && NT <= 256) ? 3 : 1) k_minimize
     float gscale = GRAD_SCALE_INIT;
     {
       double mxg = 0.0;
-      for (int i = tid; i < n; i += NT) mxg = fmax(mxg, (double)fabsf(from_fx(gacc[i]) * gscale));
+      for (int i = tid; i < n; i += NT) mxg = fmax(mxg, (double)fabsf(gread(gacc, i, m.n_atoms) * gscale));
       mxg = R.max(mxg);
       while (mxg > GRAD_CAP) {
         gscale *= 0.5f;
         mxg *= 0.5;
       }
       for (int i = tid; i < n; i += NT) {
-        VG[i] = make_float2(0.f, from_fx(gacc[i]) * gscale);
+        VG[i] = make_float2(0.f, gread(gacc, i, m.n_atoms) * gscale);
         SH[i] = make_float2(0.f, 0.f);
       }
     }
@@ -883,11 +899,13 @@ __global__ void __launch_bounds__(NT, (HM == 1 && NT <= 256) ? 3 : 1) k_minimize
       R.template run<3, 4>(v3);
       double slope = v3[1], test = v3[2];
       const double dn = sqrt(v3[0]);
-      if (dn > max_step) {  // :157-166
-        const float sc = (float)(max_step / dn);
-        for (int i = tid; i < n; i += NT) d[i] *= sc;
-        slope *= (double)sc;
-        test *= (double)sc;
+      // a step longer than max_step is scaled down (:157-166); the scale goes into the
+      // step length instead of into d, so that d stays -H g, which the next H pass uses
+      double dsc = 1.0;
+      if (dn > max_step) {
+        dsc = (double)(float)(max_step / dn);
+        slope *= dsc;
+        test *= dsc;
       }
       if (slope >= 0.0) {  // ascent bail-out :170-174
         status = 0;
@@ -899,7 +917,7 @@ __global__ void __launch_bounds__(NT, (HM == 1 && NT <= 256) ? 3 : 1) k_minimize
       bool accepted = false, have_grad = false;
       for (int ls = 0; ls < MAX_LS_ITERS; ls++) {
         if (lam < lam_min) break;
-        const float lf = (float)lam;
+        const float lf = (float)(lam * dsc);
         for (int i = tid; i < n; i += NT) {
           xs[i] = xo[i] + lf * d[i];
           if (P.ls_grad) gacc[i] = 0ull;
@@ -983,7 +1001,7 @@ __global__ void __launch_bounds__(NT, (HM == 1 && NT <= 256) ? 3 : 1) k_minimize
       n_grads++;
       double v2[2] = {0.0, 0.0};
       for (int i = tid; i < n; i += NT) {
-        const float sg = fabsf(from_fx(gacc[i]) * gscale);
+        const float sg = fabsf(gread(gacc, i, m.n_atoms) * gscale);
         v2[0] = fmax(v2[0], (double)sg);
         v2[1] = fmax(v2[1], (double)sg * fmax((double)fabsf(xs[i]), 1.0));
       }
@@ -994,7 +1012,7 @@ __global__ void __launch_bounds__(NT, (HM == 1 && NT <= 256) ? 3 : 1) k_minimize
         v2[1] *= 0.5;
       }
       for (int i = tid; i < n; i += NT) {
-        const float gn = from_fx(gacc[i]) * gscale;
+        const float gn = gread(gacc, i, m.n_atoms) * gscale;
         VG[i] = make_float2(gn - VG[i].y, gn);
       }
       if (v2[1] / fmax(E * (double)gscale, 1.0) < (double)P.grad_tol) {  // :300-316
@@ -1022,21 +1040,18 @@ __global__ void __launch_bounds__(NT, (HM == 1 && NT <= 256) ? 3 : 1) k_minimize
               const int j0 = q * seg, j1 = min(n, j0 + seg);
               const int cb = pcol(t);
               int pj = pcol(j0);
-              float a1[4] = {0.f, 0.f, 0.f, 0.f}, a2[4] = {0.f, 0.f, 0.f, 0.f};
+              float a2[4] = {0.f, 0.f, 0.f, 0.f};
               int j = j0;
               if (j + 3 < j1) {
-                float h[4];
-                float2 w[4];
+                float h[4], w[4];
                 hrow_load4(H, VG, t, cb, j, pj, h, w);
                 while (j + 7 < j1) {
-                  float hn[4];
-                  float2 wn[4];
+                  float hn[4], wn[4];
                   const int pn = pcol(j + 4);
                   hrow_load4(H, VG, t, cb, j + 4, pn, hn, wn);
   #pragma unroll
                   for (int u = 0; u < 4; u++) {
-                    a1[u] += h[u] * w[u].x;
-                    a2[u] += h[u] * w[u].y;
+                    a2[u] += h[u] * w[u];
                     h[u] = hn[u];
                     w[u] = wn[u];
                   }
@@ -1044,35 +1059,22 @@ __global__ void __launch_bounds__(NT, (HM == 1 && NT <= 256) ? 3 : 1) k_minimize
                   pj = pn;
                 }
   #pragma unroll
-                for (int u = 0; u < 4; u++) {
-                  a1[u] += h[u] * w[u].x;
-                  a2[u] += h[u] * w[u].y;
-                }
+                for (int u = 0; u < 4; u++) a2[u] += h[u] * w[u];
                 j += 4;
                 pj = pcol(j);
               }
               for (; j < j1; j++) {
                 const float hv = H[j <= t ? cb + j : pj + t];
-                const float2 wj = VG[j];
-                a1[0] += hv * wj.x;
-                a2[0] += hv * wj.y;
+                a2[0] += hv * VG[j].y;
                 pj += (j + 4) & ~3;
               }
-              float s1 = (a1[0] + a1[1]) + (a1[2] + a1[3]), s2 = (a2[0] + a2[1]) + (a2[2] + a2[3]);
-              if (lsh >= 1) {
-                s1 += dpp_f<DPP_QUAD_XOR1>(s1);
-                s2 += dpp_f<DPP_QUAD_XOR1>(s2);
-              }
-              if (lsh >= 2) {
-                s1 += dpp_f<DPP_QUAD_XOR2>(s1);
-                s2 += dpp_f<DPP_QUAD_XOR2>(s2);
-              }
-              if (lsh >= 3) {
-                s1 += dpp_f<DPP_ROW_HALF_MIRROR>(s1);
-                s2 += dpp_f<DPP_ROW_HALF_MIRROR>(s2);
-              }
+              float s2 = (a2[0] + a2[1]) + (a2[2] + a2[3]);
+              if (lsh >= 1) s2 += dpp_f<DPP_QUAD_XOR1>(s2);
+              if (lsh >= 2) s2 += dpp_f<DPP_QUAD_XOR2>(s2);
+              if (lsh >= 3) s2 += dpp_f<DPP_ROW_HALF_MIRROR>(s2);
               if (q == 0) {
-                hd[t] = s1;
+                // H dg = H g_new - H g_old = H g_new + d_old
+                hd[t] = s2 + d[t];
                 d[t] = s2;
               }
             }
@@ -1112,19 +1114,19 @@ __global__ void __launch_bounds__(NT, (HM == 1 && NT <= 256) ? 3 : 1) k_minimize
                 cy[k][u] = c.y;
               }
             }
-            float a1[4] = {0.f, 0.f, 0.f, 0.f}, a2[4] = {0.f, 0.f, 0.f, 0.f};
+            float a2[4] = {0.f, 0.f, 0.f, 0.f};
             float4* base = (float4*)H + tq;  // row j of the quad at base[j * nq]
-            float lr[4][NPM];
+            float lr[2][NPM];
 #pragma unroll
-            for (int k = 0; k < NPM; k++) lr[0][k] = lr[1][k] = lr[2][k] = lr[3][k] = 0.f;
-            hquad_dispatch<HQ, NPM>(npend, base, nq, j0, j1, VG, S, cx, cy, a1, a2, lr);
-            // the pending updates' part of H v when they were not added element by element:
-            // sum the dot products over the quad's lanes, then y_t += alpha_t (s.v) + beta_t (h.v)
+            for (int k = 0; k < NPM; k++) lr[0][k] = lr[1][k] = 0.f;
+            hquad_dispatch<HQ, NPM>(npend, base, nq, j0, j1, VG, S, cx, cy, a2, lr);
+            // the pending updates' part of H g when they were not added element by element:
+            // sum the dot products over the quad's lanes, then y_t += alpha_t (s.g) + beta_t (h.g)
             if (npend > 0 && npend < NPM) {
 #pragma unroll
               for (int k = 0; k < NPM; k++)
 #pragma unroll
-                for (int c = 0; c < 4; c++) {
+                for (int c = 0; c < 2; c++) {
                   float x = lr[c][k];
                   if (lq >= 1) x += dpp_f<DPP_QUAD_XOR1>(x);
                   if (lq >= 2) x += dpp_f<DPP_QUAD_XOR2>(x);
@@ -1134,29 +1136,18 @@ __global__ void __launch_bounds__(NT, (HM == 1 && NT <= 256) ? 3 : 1) k_minimize
             }
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-              float s1 = a1[u], s2 = a2[u];
-              if (lq >= 1) {
-                s1 += dpp_f<DPP_QUAD_XOR1>(s1);
-                s2 += dpp_f<DPP_QUAD_XOR1>(s2);
-              }
-              if (lq >= 2) {
-                s1 += dpp_f<DPP_QUAD_XOR2>(s1);
-                s2 += dpp_f<DPP_QUAD_XOR2>(s2);
-              }
-              if (lq >= 3) {
-                s1 += dpp_f<DPP_ROW_HALF_MIRROR>(s1);
-                s2 += dpp_f<DPP_ROW_HALF_MIRROR>(s2);
-              }
+              float s2 = a2[u];
+              if (lq >= 1) s2 += dpp_f<DPP_QUAD_XOR1>(s2);
+              if (lq >= 2) s2 += dpp_f<DPP_QUAD_XOR2>(s2);
+              if (lq >= 3) s2 += dpp_f<DPP_ROW_HALF_MIRROR>(s2);
               if (npend > 0 && npend < NPM) {
 #pragma unroll
-                for (int k = 0; k < NPM; k++) {
-                  s1 = fmaf(cy[k][u], lr[1][k], fmaf(cx[k][u], lr[0][k], s1));
-                  s2 = fmaf(cy[k][u], lr[3][k], fmaf(cx[k][u], lr[2][k], s2));
-                }
+                for (int k = 0; k < NPM; k++) s2 = fmaf(cy[k][u], lr[1][k], fmaf(cx[k][u], lr[0][k], s2));
               }
               const int t = 4 * tq + u;
               if (q == 0 && t < n) {
-                hd[t] = s1;
+                // H dg = H g_new - H g_old = H g_new + d_old
+                hd[t] = s2 + d[t];
                 d[t] = s2;
               }
             }
@@ -1323,7 +1314,7 @@ __global__ void __launch_bounds__(NT) k_bb_fused(KBatch B, const float* __restri
       E = E_prev;
       continue;
     }
-    for (int i = tid; i < n; i += NT) g[i] = from_fx(gacc[i]);
+    for (int i = tid; i < n; i += NT) g[i] = gread(gacc, i, m.n_atoms);
     if (it > 0) {  // BB step :1041-1052
       double v2[2] = {0.0, 0.0};
       for (int i = tid; i < n; i += NT) {
@@ -1406,7 +1397,7 @@ __global__ void __launch_bounds__(NT) k_mm_init(KBatch B, MMState S, const float
   mm_eval<NT>(B, m, xs, gacc, R, E, n);
   for (int i = threadIdx.x; i < n; i += NT) {
     S.x[off + i] = xs[i];
-    S.g[off + i] = from_fx(gacc[i]) * gsc;
+    S.g[off + i] = gread(gacc, i, m.n_atoms) * gsc;
     if (mode == 0) S.bx[off + i] = xs[i];
   }
   if (threadIdx.x == 0) {
@@ -1509,7 +1500,7 @@ __global__ void __launch_bounds__(NT) k_mm_chunk(KBatch B, MMState S, int steps,
     }
     double sy = 0.0;
     for (int i = tid; i < n; i += NT) {
-      const float gn = from_fx(gacc[i]) * gsc;
+      const float gn = gread(gacc, i, m.n_atoms) * gsc;
       const float s = xn[i] - xs[i], y = gn - gs[i];
       S.S[sl * T + off + i] = s;
       S.Y[sl * T + off + i] = y;
@@ -1714,8 +1705,7 @@ int make_nk_plan(const mmffhip_batch* b, const mmffhip_nk_opts* o, NkPlan& pl) {
     return fail(MMFFHIP_ERR_ARG, "block must be a multiple of 64 in [64, 1024]");
   pl.max_n = round4(3 * std::max(b->max_atoms, 1));
   const int m = o->algo == MMFFHIP_LBFGS ? o->lbfgs_m : 0;
-  const size_t lds_in = nk_lds_bytes(o->algo, true, pl.nt, pl.max_n, m);
-  const size_t lds_out = nk_lds_bytes(o->algo, false, pl.nt, pl.max_n, m);
+  const size_t lds_in = nk_lds_bytes(o->algo, 0, pl.nt, pl.max_n, m);
   if (o->hess_place == MMFFHIP_HESS_LDS) {
     pl.hm = 0;
   } else if (o->hess_place == MMFFHIP_HESS_GLOBAL) {
@@ -1727,14 +1717,14 @@ int make_nk_plan(const mmffhip_batch* b, const mmffhip_nk_opts* o, NkPlan& pl) {
   } else {
     return fail(MMFFHIP_ERR_ARG, "unknown hess_place");
   }
-  pl.lds = pl.hm == 0 ? lds_in : lds_out;
+  pl.lds = nk_lds_bytes(o->algo, pl.hm, pl.nt, pl.max_n, m);
   if (pl.lds > (size_t)di.lds) return fail(MMFFHIP_ERR_SIZE, "plan exceeds the LDS of one workgroup");
   // Stage term tables in LDS when asked and when they fit: all of them next to an LDS
   // Hessian; only the bonded ones beside a global Hessian, and only when that keeps the
   // unstaged kernel's workgroups per CU (there, occupancy hides the pair loads).
   if (o->table_lds_bytes > 0) {
     const int tl = pl.hm == 0 ? 1 : 2;
-    const size_t with = nk_lds_bytes(o->algo, pl.hm == 0, pl.nt, pl.max_n, m, (size_t)o->table_lds_bytes);
+    const size_t with = nk_lds_bytes(o->algo, pl.hm, pl.nt, pl.max_n, m, (size_t)o->table_lds_bytes);
     const void* kt = nk_kernel(o->algo, pl.hm, tl, pl.nt);
     bool take = with <= (size_t)di.lds && kt != nullptr;
     if (take && tl == 2) {
@@ -1783,6 +1773,38 @@ void launch_nk(int algo, const KBatch& kb, const NkParams& P, const NkPlan& pl, 
   }
   NK_FOR_EACH(NKL)
 #undef NKL
+}
+
+// Clear the work queue at the front of `workspace` with a fill kernel (graph-capture safe)
+// and launch the plan's minimize kernel; the global slots follow the 256-byte queue header.
+int run_nk(const mmffhip_batch* b, const mmffhip_nk_opts* o, const NkPlan& pl, const float* x_in, float* x_out,
+           double* e_out, int32_t* status_out, int32_t* counters, const int32_t* order, void* workspace,
+           hipStream_t s) {
+  int32_t* queue = (int32_t*)workspace;
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, queue, 1, 0);
+  HIP_TRY(hipGetLastError());
+  NkParams P;
+  P.x_in = x_in;
+  P.x_out = x_out;
+  P.e_out = e_out;
+  P.status = status_out;
+  P.counters = counters;
+  P.order = order;
+  P.queue = queue;
+  P.hglob = (float*)((char*)workspace + 256);
+  P.slot_floats = pl.slot_floats;
+  P.precond = o->precond;
+  P.max_iters = o->max_iters;
+  P.grad_tol = o->grad_tol;
+  P.diel = o->diel;
+  P.m = o->algo == MMFFHIP_LBFGS ? o->lbfgs_m : 1;
+  P.max_n = pl.max_n;
+  P.ls_grad = o->ls_grad;
+  P.tbytes = pl.tbytes;
+  KBatch kb = to_kbatch(b);
+  launch_nk(o->algo, kb, P, pl, s);
+  HIP_TRY(hipGetLastError());
+  return MMFFHIP_OK;
 }
 
 }  // namespace
@@ -1851,32 +1873,7 @@ int mmffhip_minimize_nk(const mmffhip_batch* b, const mmffhip_nk_opts* o, const 
   if (rc) return rc;
   if (workspace_bytes < nk_ws_bytes(pl)) return fail(MMFFHIP_ERR_SIZE, "workspace too small");
   if ((uintptr_t)workspace & 255) return fail(MMFFHIP_ERR_ARG, "workspace must be 256-byte aligned");
-  hipStream_t s = (hipStream_t)stream;
-  int32_t* queue = (int32_t*)workspace;
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, queue, 1, 0);
-  HIP_TRY(hipGetLastError());
-  NkParams P;
-  P.x_in = x_in;
-  P.x_out = x_out;
-  P.e_out = e_out;
-  P.status = status_out;
-  P.counters = counters;
-  P.order = order;
-  P.queue = queue;
-  P.hglob = (float*)((char*)workspace + 256);
-  P.slot_floats = pl.slot_floats;
-  P.precond = o->precond;
-  P.max_iters = o->max_iters;
-  P.grad_tol = o->grad_tol;
-  P.diel = o->diel;
-  P.m = o->algo == MMFFHIP_LBFGS ? o->lbfgs_m : 1;
-  P.max_n = pl.max_n;
-  P.ls_grad = o->ls_grad;
-  P.tbytes = pl.tbytes;
-  KBatch kb = to_kbatch(b);
-  launch_nk(o->algo, kb, P, pl, s);
-  HIP_TRY(hipGetLastError());
-  return MMFFHIP_OK;
+  return run_nk(b, o, pl, x_in, x_out, e_out, status_out, counters, order, workspace, (hipStream_t)stream);
 }
 
 int mmffhip_bb_fused(const mmffhip_batch* b, const float* x_in, float* x_out, double* e_out, int32_t max_iters,

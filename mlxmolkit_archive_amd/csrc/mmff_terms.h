@@ -331,8 +331,9 @@ __device__ __forceinline__ float t_torsion(const float* x, Acc& acc, int i1, int
 // electrostatics (TG :368-402, MEGA :490-502). qqs = q_i q_j * (0.75 for 1-4 pairs, else 1);
 // eps = 0 / qqs = 0 switch a half off. diel = 1 (constant) or 2 (distance dependent, TG only).
 // Returns vdW energy in ev and ele energy in ee.
+// iR = 1 / R* (1 / max(R*, 1e-8) for the TG family), precomputed in the pair table.
 template <int FAM, bool G, class Acc>
-__device__ __forceinline__ void t_pair(const float* x, Acc& acc, int i, int j, float R, float eps, float qqs,
+__device__ __forceinline__ void t_pair(const float* x, Acc& acc, int i, int j, float iR, float eps, float qqs,
                                        int diel, float& ev, float& ee) {
   f3 d = ld(x, i) - ld(x, j);
   float d2 = dot(d, d);
@@ -342,7 +343,6 @@ __device__ __forceinline__ void t_pair(const float* x, Acc& acc, int i, int j, f
     const float d2c = fmaxf(d2, 1e-16f);
     const float ir = rsqrtf(d2c);
     const float r = d2c * ir;
-    const float iR = rcp(fmaxf(R, 1e-8f));
     const float q = r * iR;
     const float q2 = q * q, q6 = q2 * q2 * q2, q7 = q6 * q;
     const float rq = rcp(q + 0.07f);
@@ -362,9 +362,8 @@ __device__ __forceinline__ void t_pair(const float* x, Acc& acc, int i, int j, f
       acc.add_pair(i, j, f * d.x, f * d.y, f * d.z);
     }
   } else {
-    // shared reciprocals: 1/R, 1/(rho + 0.07), 1/(rho^7 + 0.12), 1/(r + 0.05), 1/r
+    // shared reciprocals: 1/(rho + 0.07), 1/(rho^7 + 0.12), 1/(r + 0.05), 1/r
     const float r = sqrtf(d2) + M_KEPS;
-    const float iR = rcp(R);
     const float rho = r * iR;
     const float rho2 = rho * rho, rho6 = rho2 * rho2 * rho2, rho7 = rho6 * rho;
     const float rq = rcp(rho + 0.07f);

@@ -146,11 +146,13 @@ class NkRunner:
         self.st = torch.empty(C_, dtype=torch.int32, device=dev)
         self.cnt = torch.empty((C_, 3), dtype=torch.int32, device=dev)
 
-    def run(self, x_in, x_out=None, stream=None, overlap: bool = True) -> NkResult:
+    def run(self, x_in, x_out=None, stream=None, overlap: bool = True, events=None) -> NkResult:
         """Minimize every conformer of the plan. With ``overlap`` the size-bucket launches go to
         up to three side streams (forked from and joined back to ``stream``), so that the
         HBM-bound large-molecule bucket runs beside the LDS-bound ones and launch tails fill in;
-        the launches write disjoint conformers. Results are the same as sequential launches."""
+        the launches write disjoint conformers. Results are the same as sequential launches.
+        ``events``: optional list of (start, end) torch.cuda.Event pairs, one per launch,
+        recorded around each launch on the stream it runs on (per-launch kernel time)."""
         torch = _lib.require_gpu()
         assert x_in.dtype == torch.float32 and x_in.numel() == self.db.host.total_coords
         if x_out is None:
@@ -166,9 +168,13 @@ class NkRunner:
                 s.wait_stream(main)
         for k, ln in enumerate(self.launches):
             s = streams[k % len(streams)]
+            if events is not None:
+                events[k][0].record(s)
             _lib.check(L.mmffhip_minimize_nk(C.byref(ln["desc"]), C.byref(ln["opts"]), _p(x_in), _p(x_out),
                                              _p(self.e), _p(self.st), _p(self.cnt), _p(ln["order"]), _p(ln["ws"]),
                                              ln["ws_bytes"], int(s.cuda_stream)))
+            if events is not None:
+                events[k][1].record(s)
         if streams[0] is not main:
             for s in streams:
                 main.wait_stream(s)

@@ -126,6 +126,9 @@ class _RDKitSource:
     def is_linear(self, i):
         return self.mol.GetAtomWithIdx(i).GetHybridization() == self.Chem.HybridizationType.SP
 
+    def atom_type(self, i):
+        return int(self.mp.GetMMFFAtomType(i))
+
     def bond(self, i, j):
         r = self.mp.GetMMFFBondStretchParams(self.mol, i, j)
         return None if not r else (r[1], r[2])
@@ -183,6 +186,9 @@ class _OwnSource:
     def is_linear(self, i):
         return self.t.is_linear(i)
 
+    def atom_type(self, i):
+        return int(self.t.types[i])
+
     def bond(self, i, j):
         return self.t.bond(i, j)
 
@@ -227,13 +233,33 @@ def _positions(mol, conf_id: int) -> np.ndarray:
     return out
 
 
+# MMFF atom types whose angles RDKit treats as linear (its MMFFProp::linh flag), the rule the
+# reference's second extractor follows (mlxmolkit/mmff_extract_shivam.py:16-17,127-129)
+MMFF_LINEAR_TYPES = frozenset({4, 53, 61})
+
+
 def extract_mmff_params(mol, conf_id: int = 0, non_bonded_thresh: float = 100.0,
-                        mmff_variant: str = "MMFF94") -> MMFFParams:
+                        mmff_variant: str = "MMFF94", *, linear_rule: str = "hybridization",
+                        ignore_interfrag_interactions: bool = False) -> MMFFParams:
     """Extract MMFF94 or MMFF94s parameters (same signature as the reference :100-105).
+
+    Keyword-only opt-ins (defaults reproduce mlxmolkit/mmff_params.py):
+
+    * ``linear_rule``: which angle centres take the linear bend form and lose their
+      stretch-bend terms. ``"hybridization"`` (default): sp centres, as mmff_params.py:168
+      does; ``"mmff_types"``: centres of MMFF type 4, 53 or 61, RDKit's own rule
+      (mmff_extract_shivam.py:127-129,204,230-238).
+    * ``ignore_interfrag_interactions``: drop vdW and electrostatic pairs between
+      disconnected fragments (mmff_extract_shivam.py:361-378; RDKit's MMFF default). The
+      default keeps them, as mmff_params.py:284-309 does.
 
     Raises ``ValueError`` when the molecule cannot be typed, as the reference does (:123-124).
     """
+    if linear_rule not in ("hybridization", "mmff_types"):
+        raise ValueError(f"linear_rule must be 'hybridization' or 'mmff_types', not {linear_rule!r}")
     src = _RDKitSource(mol, mmff_variant) if _is_rdkit_mol(mol) else _OwnSource(mol, mmff_variant)
+    if linear_rule == "mmff_types":
+        src.is_linear = lambda i: src.atom_type(i) in MMFF_LINEAR_TYPES
     n = src.n_atoms()
     p = MMFFParams(n_atoms=n)
 
@@ -327,6 +353,9 @@ def extract_mmff_params(mol, conf_id: int = 0, non_bonded_thresh: float = 100.0,
     pos = _positions(mol, conf_id)
     iu, ju = np.triu_indices(n, 1)
     keep = rel[iu, ju] >= RELATION_1_4
+    if ignore_interfrag_interactions:
+        # unreachable pairs (different fragments) have no finite bond path
+        keep &= dm[iu, ju] < n
     if keep.any():
         d = np.linalg.norm(pos[iu] - pos[ju], axis=1)
         keep &= d <= non_bonded_thresh

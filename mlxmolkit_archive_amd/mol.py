@@ -239,6 +239,11 @@ def mol_from_smiles(smiles: str, add_hs: bool = True, name: str = "") -> Mol:
             pend_order = {"-": 1.0, "=": 2.0, "#": 3.0, ":": 1.5}[ch]
             i += 1
             continue
+        if ch == ".":  # a new disconnected fragment: the next atom bonds to nothing
+            prev = -1
+            pend_order = None
+            i += 1
+            continue
         if ch.isdigit() or ch == "%":
             if ch == "%":
                 num = int(s[i + 1:i + 3])

@@ -169,7 +169,9 @@ def pack(params_list: Sequence, conf_counts: Sequence[int]) -> HostBatch:
         parts["tor_p"].append(np.stack([p.torsion_V1, p.torsion_V2, p.torsion_V3, np.zeros(nt, np.float32)],
                                        1).astype(np.float32).reshape(-1, 4))
         parts["pair_ij"].append(_pack16(pi, pj))
-        parts["pair_p"].append(np.stack([R, eps, qq, np.zeros(npair, np.float32)], 1).astype(np.float32))
+        # w = 1 / R*, rounded once here, so the pair loop needs no reciprocal of a table constant
+        inv_r = (1.0 / np.maximum(np.asarray(R, np.float64), 1e-8)).astype(np.float32)
+        parts["pair_p"].append(np.stack([R, eps, qq, inv_r], 1).astype(np.float32))
 
     def cat(k, shape_tail, dtype):
         arrs = [a for a in parts[k] if a.size]

@@ -279,7 +279,9 @@ def test_multimol_matches_oracle(small_wl):
 def test_metal_batch_form_matches_oracle(small_wl):
     """The mmff_optimize_metal_batch form (no convergence exit, no polish pass, L2 norm of the
     last scaled gradient) against the oracle's batch form: after 20 iterations the geometries,
-    energies and norms agree; after 200 the best-seen energies stay close."""
+    energies and norms agree; after 200 iterations, with the trajectories still moving, the
+    best-seen energies are as close to the oracle's as the oracle's own are to a restart of it
+    from 1e-6 A away (a measure of how fast nearby trajectories separate)."""
     eng = _eng()
     wl = small_wl
     db = eng.device_batch(wl.params, wl.conf_counts)
@@ -300,7 +302,11 @@ def test_metal_batch_form_matches_oracle(small_wl):
             assert np.abs(e - eo).max() < 1e-3 + 1e-5 * np.abs(eo).max()
             assert np.allclose(gn, gno, rtol=1e-2, atol=1e-5)
         else:
-            assert np.median(np.abs(e - eo)) < 0.01
+            xp = wl.x.astype(np.float64) + np.random.default_rng(5).normal(0.0, 1e-6, wl.x.shape)
+            ep = oracle.multi_mol(ob, xp, max_iters=iters, grad_tol=0.0, batch_mode=True)[1]
+            spread = float(np.median(np.abs(ep - eo)))
+            print(f"median |dE| gpu vs oracle {np.median(np.abs(e - eo)):.3e}, oracle vs its restart {spread:.3e}")
+            assert np.median(np.abs(e - eo)) < max(0.01, 3.0 * spread)
 
 
 def test_empty_batch_is_a_no_op():

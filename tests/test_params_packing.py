@@ -154,3 +154,41 @@ def test_bucket_tables_cover_every_size_and_name_known_placements():
     bonded = engine.table_lds_bytes(hb, pairs=False)
     npair = hb.mol_desc.reshape(-1, 16)[:, 12].astype(np.int64)
     assert np.array_equal(full - bonded, 20 * npair)
+
+
+def test_ignore_interfrag_interactions_opt_in():
+    """Default (mmff_params.py:284-309): pairs between disconnected fragments are kept;
+    ignore_interfrag_interactions=True (mmff_extract_shivam.py:361-378) drops exactly those."""
+    m = mol_from_smiles("CCO.O")
+    m.set_conformers(np.zeros((1, m.GetNumAtoms(), 3)))
+    a = extract_mmff_params(m)
+    b = extract_mmff_params(m, ignore_interfrag_interactions=True)
+    frag = np.full(m.GetNumAtoms(), -1)
+    dm = m.topological_distances()
+    for i in range(m.GetNumAtoms()):
+        if frag[i] < 0:
+            frag[dm[i] < m.GetNumAtoms()] = frag.max() + 1
+    assert frag.max() == 1
+    cross = lambda p, q: frag[p] != frag[q]  # noqa: E731
+    assert cross(a.vdw_idx1, a.vdw_idx2).any()
+    assert not cross(b.vdw_idx1, b.vdw_idx2).any() and not cross(b.ele_idx1, b.ele_idx2).any()
+    keep = ~cross(a.vdw_idx1, a.vdw_idx2)
+    assert np.array_equal(a.vdw_idx1[keep], b.vdw_idx1) and np.array_equal(a.vdw_R_star[keep], b.vdw_R_star)
+    keep_e = ~cross(a.ele_idx1, a.ele_idx2)
+    assert np.array_equal(a.ele_charge_term[keep_e], b.ele_charge_term)
+
+
+@pytest.mark.parametrize("smi", ["CC#N", "C=C=C", "O=C=O", "C#CC(C)C=C"])
+def test_linear_rule_mmff_types_opt_in(smi):
+    """linear_rule='mmff_types' flags exactly the angle centres of MMFF type 4, 53 or 61
+    (mmff_extract_shivam.py:16-17,204) and drops the stretch-bends centred on them (:230-238)."""
+    from mlxmolkit_archive_amd.mmff_params import MMFF_LINEAR_TYPES, _OwnSource
+    m = mol_from_smiles(smi)
+    m.set_conformers(np.zeros((1, m.GetNumAtoms(), 3)))
+    p = extract_mmff_params(m, linear_rule="mmff_types")
+    types = np.asarray(_OwnSource(m, "MMFF94").t.types)
+    want = np.isin(types[p.angle_idx2], list(MMFF_LINEAR_TYPES)).astype(np.int32)
+    assert np.array_equal(p.angle_is_linear, want)
+    assert not np.isin(types[p.strbend_idx2], list(MMFF_LINEAR_TYPES)).any()
+    with pytest.raises(ValueError):
+        extract_mmff_params(m, linear_rule="bogus")
