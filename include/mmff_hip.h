@@ -132,6 +132,21 @@ int mmffhip_minimize_nk(const mmffhip_batch *b, const mmffhip_nk_opts *o, const 
                         double *e_out, int32_t *status_out, int32_t *counters, const int32_t *order,
                         void *workspace, size_t workspace_bytes, void *stream);
 
+/* mmffhip_minimize_nk without a workspace argument: the work queue and the global Hessian /
+ * L-BFGS history slots live in a 64 MiB device array inside the library (one per GPU, held
+ * until the process ends), so the call allocates nothing and needs no caller buffer beyond
+ * inputs and outputs. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). The persistent grid is capped to the slots that fit; a plan that needs
+ * more than 64 MiB for one slot per compute unit (a full global Hessian above n = 3N = 256 on
+ * 256 CUs) fails with MMFFHIP_ERR_SIZE. Results are identical to mmffhip_minimize_nk's.
+ * Every call on one GPU shares that GPU's array, which keeps the last call's contents: calls
+ * that overlap on one GPU (concurrent streams or threads) race and must be serialised by the
+ * caller. */
+int mmffhip_minimize_nk_noalloc(const mmffhip_batch *b, const mmffhip_nk_opts *o, const float *x_in, float *x_out,
+                                double *e_out, int32_t *status_out, int32_t *counters, const int32_t *order,
+                                void *stream);
+
 /* Barzilai-Borwein steepest descent of mmff_optimize_metal_fused (MEGA family).
  * e_out: energy of the last pre-step geometry, as the reference returns. */
 int mmffhip_bb_fused(const mmffhip_batch *b, const float *x_in, float *x_out, double *e_out, int32_t max_iters,

@@ -17,6 +17,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 
 #include "mmff_hip.h"
@@ -1807,6 +1808,33 @@ int run_nk(const mmffhip_batch* b, const mmffhip_nk_opts* o, const NkPlan& pl, c
   return MMFFHIP_OK;
 }
 
+// Scratch of the allocation-free minimize entry point: the work-queue header and the
+// per-workgroup global slots live in this file-scope device array (one copy per GPU, held
+// until the process ends). Host code never passes `scratch` itself to a kernel: its address
+// on each device comes from hipGetSymbolAddress, taken once per device on that device's
+// first call.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)64 << 20;
+__device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+void* g_scratch_ptr[MAX_DEVICES];
+std::mutex g_scratch_mu;
+
+int scratch_ptr(void** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEVICES) return fail(MMFFHIP_ERR_DEVICE, "device ordinal out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(MMFFHIP_ERR_HIP, "hipGetSymbolAddress returned NULL for the scratch array");
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return MMFFHIP_OK;
+}
+
 }  // namespace
 
 // error reporting shared with the library's other translation units (not exported)
@@ -1874,6 +1902,36 @@ int mmffhip_minimize_nk(const mmffhip_batch* b, const mmffhip_nk_opts* o, const 
   if (workspace_bytes < nk_ws_bytes(pl)) return fail(MMFFHIP_ERR_SIZE, "workspace too small");
   if ((uintptr_t)workspace & 255) return fail(MMFFHIP_ERR_ARG, "workspace must be 256-byte aligned");
   return run_nk(b, o, pl, x_in, x_out, e_out, status_out, counters, order, workspace, (hipStream_t)stream);
+}
+
+int mmffhip_minimize_nk_noalloc(const mmffhip_batch* b, const mmffhip_nk_opts* o, const float* x_in, float* x_out,
+                                double* e_out, int32_t* status_out, int32_t* counters, const int32_t* order,
+                                void* stream) {
+  int rc = check_batch(b);
+  if (rc) return rc;
+  if (b->n_confs == 0) return MMFFHIP_OK;
+  if (!x_in || !x_out || !e_out || !status_out)
+    return fail(MMFFHIP_ERR_ARG, "x_in, x_out, e_out and status_out are required");
+  NkPlan pl;
+  rc = make_nk_plan(b, o, pl);
+  if (rc) return rc;
+  // The persistent grid is capped to the global slots that fit beside the queue header; a
+  // plan needs at least one slot per compute unit (fewer would leave CUs idle), else the
+  // batch's molecules are too large for the array and the call is refused.
+  if (pl.slot_floats > 0) {
+    const size_t slot_bytes = pl.slot_floats * 4;
+    const size_t fit = (NOALLOC_SCRATCH_BYTES - 256) / slot_bytes;
+    const size_t need = (size_t)std::min(pl.grid, dev_info().n_cu);
+    if (fit < need)
+      return fail(MMFFHIP_ERR_SIZE, "mmffhip_minimize_nk_noalloc: " + std::to_string(need) + " slots of " +
+                                        std::to_string(slot_bytes) + " bytes (one per compute unit) exceed its 64 MiB "
+                                        "scratch; use mmffhip_minimize_nk with a workspace, or a smaller largest molecule");
+    pl.grid = (int)std::min<size_t>((size_t)pl.grid, fit);
+  }
+  void* ws = nullptr;
+  rc = scratch_ptr(&ws);
+  if (rc) return rc;
+  return run_nk(b, o, pl, x_in, x_out, e_out, status_out, counters, order, ws, (hipStream_t)stream);
 }
 
 int mmffhip_bb_fused(const mmffhip_batch* b, const float* x_in, float* x_out, double* e_out, int32_t max_iters,
