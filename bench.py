@@ -86,6 +86,8 @@ def main():
     ap.add_argument("--sample", type=int, default=2048, help="conformers checked by the FP64 oracle")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--embedder", default="device", choices=["device", "host"],
+                    help="start geometries: the restraint embedder on the GPU (default) or its host C form")
     ap.add_argument("--graph", action="store_true",
                     help="replay the step from a HIP graph (then only whole-step times are measured)")
     ap.add_argument("--engine", default="nk", choices=["nk", "multimol"],
@@ -104,10 +106,10 @@ def main():
     import torch
     from mlxmolkit_archive_amd import _lib, engine, workloads
 
-    # the workload is built before the GPU is touched: large workloads type their molecules
-    # in forked worker processes
+    # molecules are typed before the GPU is touched (large workloads type them in forked
+    # worker processes); the device embedder then places the start geometries
     t0 = time.perf_counter()
-    wl = workloads.build(args.workload, n_mols=args.mols, n_confs=args.confs)
+    wl = workloads.build(args.workload, n_mols=args.mols, n_confs=args.confs, embedder=args.embedder)
     t_gen = time.perf_counter() - t0
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.engine == "multimol":
@@ -272,14 +274,15 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32 terms, f64 energy sums",
-        "data": "synthetic: seeded molecules, typed by the built-in MMFF94-style typer, embedded + N(0,0.1A) noise",
+        "data": f"synthetic: seeded molecules, typed by the built-in MMFF94-style typer, embedded ({args.embedder} "
+                "restraint embedder) + N(0,0.1A) noise",
         "config": {"workload": f"{args.workload}: {len(wl.params)} molecules x {wl.conf_counts[0]} conformers",
                    "algorithm": (f"L-BFGS m={args.lbfgs_m}" if args.lbfgs else "dense BFGS")
                                 + (" from learned H0" if pc is not None else ""), "max_iters": args.max_iters,
                    "grad_tol": args.grad_tol, "atoms_min_max": [int(wl.atoms_per_conf().min()),
                                                                 int(wl.atoms_per_conf().max())],
                    "parallelism": "1 GPU, one workgroup per conformer, persistent size-bucketed launches",
-                   "launch": "eager, one launch per size bucket on side streams" if graph is None
+                   "launch": "eager, one launch per size bucket, in sequence" if graph is None
                              else "HIP graph replay"},
         "mean_iters_to_converge": round(mean_iters, 1),
         "status0_fraction": round(n_status0 / wl.n_confs, 4),

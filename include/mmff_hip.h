@@ -173,6 +173,18 @@ int mmffhip_multimol_run_ex(const mmffhip_batch *b, const float *x_in, float *x_
                             float grad_scale, float grad_tol, int32_t flags, void *workspace,
                             size_t workspace_bytes, void *stream);
 
+/* Start geometries on the device (restraint embedding, FP64): stands in for the reference
+ * pipeline's device DG -> 4D->3D -> ETK stages (mlxmolkit/conformer_pipeline_v2.py:176-290),
+ * so the minimizer's inputs are produced where it reads them. mol_desc: 4 int32 per molecule
+ * {n_atoms, restraint offset, restraint count, 0}; restr_ij: i | j<<16; restr_d: target
+ * distance d0 > 0 of an equality spring 100 (d - d0)^2, or -d_min for a lower-bound wall
+ * 10 (d_min - d)^2 when d < d_min. Conformer c embeds molecule conf_mol[c] from
+ * seeds[c] (splitmix64 stream) and writes its 3N floats at x_out + conf_off[c], with
+ * N(0, noise) added to every coordinate. */
+int mmffhip_embed(const int32_t *mol_desc, const uint32_t *restr_ij, const float *restr_d, const int32_t *conf_mol,
+                  const int32_t *conf_off, const uint64_t *seeds, int32_t n_confs, int32_t max_atoms, float noise,
+                  float *x_out, void *stream);
+
 /* Learned initial inverse Hessian: a per-atom MLP (104 features -> 64 -> 64 -> 6, ReLU) whose 6
  * outputs are the Cholesky factor of a 3x3 SPD block (diagonal through softplus + 1e-3). No
  * counterpart in the reference; it produces the optional mmffhip_nk_opts.precond blocks.

@@ -1,7 +1,8 @@
 """In-tree build of the package's native libraries.
 
 * ``libmmff_hip.so``   hand-written HIP kernels for gfx950 + the extern "C" ABI
-                       (csrc/mmff_hip.hip, csrc/precond.hip; declared in include/mmff_hip.h)
+                       (csrc/mmff_hip.hip, csrc/precond.hip, csrc/embed.hip; declared in
+                       include/mmff_hip.h)
 * ``libmmff_embed.so`` host-side start-geometry embedder (csrc/embed.c)
 
 Built in place next to this file so that the libraries travel with the source tree.
@@ -58,7 +59,7 @@ def build_hip() -> Path:
     out = lib_path("libmmff_hip.so")
     tmp = out.with_suffix(".so.tmp")
     _run([hipcc(), *HIP_FLAGS, "-I", str(INCLUDE), "-I", str(CSRC), str(CSRC / "mmff_hip.hip"),
-          str(CSRC / "precond.hip"), "-o", str(tmp)])
+          str(CSRC / "precond.hip"), str(CSRC / "embed.hip"), "-o", str(tmp)])
     os.replace(tmp, out)
     return out
 

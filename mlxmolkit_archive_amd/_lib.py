@@ -56,6 +56,7 @@ SIGNATURES = {
                                        C.c_int32, C.c_float, C.c_float, _vp, C.c_size_t, _vp]),
     "mmffhip_multimol_run_ex": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_int32, C.c_int32, C.c_float,
                                           C.c_int32, C.c_float, C.c_float, C.c_int32, _vp, C.c_size_t, _vp]),
+    "mmffhip_embed": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int32, C.c_int32, C.c_float, _vp, _vp]),
     "mmffhip_mlp_dims": (C.c_int, [_vp, _vp, _vp, _vp]),
     "mmffhip_mlp_infer": (C.c_int, [_vp, _vp, C.c_int32, _vp, _vp]),
     "mmffhip_mlp_workspace": (C.c_int, [C.c_int32, _vp]),

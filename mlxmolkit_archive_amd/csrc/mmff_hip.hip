@@ -125,8 +125,9 @@ constexpr float FX_CAP = 8388000.0f;      // |v| is capped just below 2^23
 constexpr double FX_UNIT = 3.7252902984619140625e-09;  // 2^-28
 
 __device__ __forceinline__ unsigned long long to_fx(float v) {
-  // NaN maps to 0
-  const float c = (v == v) ? fminf(fmaxf(v, -FX_CAP), FX_CAP) : 0.0f;
+  // fmaxf / fminf return the non-NaN operand (one v_med3_f32), so a NaN contribution maps to
+  // -FX_CAP: finite, and only ever met where the energy is NaN too
+  const float c = fminf(fmaxf(v, -FX_CAP), FX_CAP);
   const double y = (double)c + FX_MAGIC;
   return (unsigned long long)(__double_as_longlong(y) - __double_as_longlong(FX_MAGIC));
 }
@@ -137,7 +138,8 @@ __device__ __forceinline__ float from_fx(unsigned long long a) {
 
 // The accumulators of a conformer with N atoms are three planes of N words (x, y, z), so
 // lanes updating consecutive atoms touch consecutive 8-byte words (no bank conflicts); the
-// gradient of coordinate i = 3a + c is plane c, word a.
+// gradient of coordinate i = 3a + c is plane c, word a. The -g side of a pair term is an LDS
+// subtract (ds_sub_u64) of the same integer, so it needs no negation.
 template <bool G>
 struct LdsAcc {
   unsigned long long* g;
@@ -149,16 +151,19 @@ struct LdsAcc {
       atomicAdd(&g[2 * N + a], to_fx(z));
     }
   }
-  // +g on atom a, -g on atom b: one conversion serves both (to_fx(-x) == -to_fx(x))
+  __device__ __forceinline__ static void sub(unsigned long long* p, unsigned long long v) {
+    __hip_atomic_fetch_sub(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
+  // +g on atom a, -g on atom b: one conversion serves both
   __device__ __forceinline__ void add_pair(int a, int b, float x, float y, float z) {
     if (G) {
       const unsigned long long fx = to_fx(x), fy = to_fx(y), fz = to_fx(z);
       atomicAdd(&g[a], fx);
       atomicAdd(&g[N + a], fy);
       atomicAdd(&g[2 * N + a], fz);
-      atomicSub(&g[b], fx);
-      atomicSub(&g[N + b], fy);
-      atomicSub(&g[2 * N + b], fz);
+      sub(&g[b], fx);
+      sub(&g[N + b], fy);
+      sub(&g[2 * N + b], fz);
     }
   }
 };
@@ -503,36 +508,27 @@ __device__ __forceinline__ void hquad_dot(const float4& h, float g, float (&a)[4
   a[3] = fmaf(h.w, g, a[3]);
 }
 
-// Rows [j, j1) of a column quad of the full Hessian (row r at base[r * nq]): accumulate
-// H g into a, with H the stored rows plus the NU oldest pending rank-2 updates (update k:
-// row vectors {s, h} in S[k], column coefficients cx[k], cy[k]). When all NPM slots are
-// pending the updates are added to each element, oldest first, and the rows are written
-// back. Otherwise the stored rows are only read, and the updates' part of H g is left as two
-// dot products per update over these rows in lr (s.g, h.g), which the caller combines with
-// the column coefficients after summing the quad's lanes. HQ rows are loaded together.
-template <int HQ, int NU, int NPM>
+// Rows [j, j1) of a column quad of the full Hessian (row r at base[r * nq]): accumulate the
+// stored rows' part of H g into a. With FLUSH, all NPM pending rank-2 updates (update k: row
+// vectors {s, h} in S[k], column coefficients cx[k], cy[k]) are first added to each element,
+// oldest first, and the rows are written back; without it the rows are only read (the
+// pending updates enter H g through dot products taken once per iteration). HQ rows are
+// loaded together.
+template <int HQ, bool FLUSH, int NPM>
 __device__ __forceinline__ void hquad_rows(float4* base, int nq, int j, int j1, const float2* VG,
                                            const float2* const (&S)[NPM], const float (&cx)[NPM][4],
-                                           const float (&cy)[NPM][4], float (&a)[4], float (&lr)[2][NPM]) {
-  constexpr bool FLUSH = NU == NPM;
+                                           const float (&cy)[NPM][4], float (&a)[4]) {
   for (; j + HQ - 1 < j1; j += HQ) {
     float4 h[HQ];
 #pragma unroll
     for (int u = 0; u < HQ; u++) h[u] = base[(size_t)(j + u) * nq];
 #pragma unroll
     for (int u = 0; u < HQ; u++) {
-      const float g = VG[j + u].y;
+      if (FLUSH) {
 #pragma unroll
-      for (int k = 0; k < NU; k++) {
-        const float2 w = S[k][j + u];
-        if (FLUSH) {
-          hquad_upd(h[u], w, cx[k], cy[k]);
-        } else {
-          lr[0][k] = fmaf(w.x, g, lr[0][k]);
-          lr[1][k] = fmaf(w.y, g, lr[1][k]);
-        }
+        for (int k = 0; k < NPM; k++) hquad_upd(h[u], S[k][j + u], cx[k], cy[k]);
       }
-      hquad_dot(h[u], g, a);
+      hquad_dot(h[u], VG[j + u].y, a);
     }
     if (FLUSH) {
 #pragma unroll
@@ -541,34 +537,12 @@ __device__ __forceinline__ void hquad_rows(float4* base, int nq, int j, int j1, 
   }
   for (; j < j1; j++) {
     float4 h = base[(size_t)j * nq];
-    const float g = VG[j].y;
+    if (FLUSH) {
 #pragma unroll
-    for (int k = 0; k < NU; k++) {
-      const float2 w = S[k][j];
-      if (FLUSH) {
-        hquad_upd(h, w, cx[k], cy[k]);
-      } else {
-        lr[0][k] = fmaf(w.x, g, lr[0][k]);
-        lr[1][k] = fmaf(w.y, g, lr[1][k]);
-      }
+      for (int k = 0; k < NPM; k++) hquad_upd(h, S[k][j], cx[k], cy[k]);
     }
-    hquad_dot(h, g, a);
+    hquad_dot(h, VG[j].y, a);
     if (FLUSH) base[(size_t)j * nq] = h;
-  }
-}
-
-// hquad_rows for the run-time count of pending updates (0..NPM)
-template <int HQ, int NPM, int NU = NPM>
-__device__ __forceinline__ void hquad_dispatch(int npend, float4* base, int nq, int j, int j1, const float2* VG,
-                                               const float2* const (&S)[NPM], const float (&cx)[NPM][4],
-                                               const float (&cy)[NPM][4], float (&a)[4], float (&lr)[2][NPM]) {
-  if constexpr (NU == 0) {
-    hquad_rows<HQ, 0, NPM>(base, nq, j, j1, VG, S, cx, cy, a, lr);
-  } else {
-    if (npend == NU)
-      hquad_rows<HQ, NU, NPM>(base, nq, j, j1, VG, S, cx, cy, a, lr);
-    else
-      hquad_dispatch<HQ, NPM, NU - 1>(npend, base, nq, j, j1, VG, S, cx, cy, a, lr);
   }
 }
 
@@ -645,10 +619,8 @@ __global__ void __launch_bounds__(NT) k_energy_grad(KBatch B, const float* __res
   const MolDesc m = load_desc(B.mol_desc + 16 * B.conf_mol[c]);
   const int n = 3 * m.n_atoms;
   const int off = B.conf_off[c];
-  for (int i = threadIdx.x; i < n; i += NT) {
-    xs[i] = x[off + i];
-    gacc[i] = 0ull;
-  }
+  for (int i = threadIdx.x; i < n; i += NT) xs[i] = x[off + i];
+  for (int i = threadIdx.x; i < n; i += NT) gacc[i] = 0ull;
   __syncthreads();
   Reducer<NT> R{red, 0};
   double et[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -726,7 +698,7 @@ __host__ __device__ inline size_t nk_lds_bytes(int algo, int hm, int nt, int max
   const bool hlds = hm == 0;
   size_t nw = (size_t)nt / 64;
   size_t b = 16 * nw * sizeof(double) + 64 + 2 * (size_t)LBFGS_MAX_M * sizeof(double);
-  b += (size_t)max_n * 8;       // gradient accumulators
+  b += (size_t)max_n * 8;  // gradient accumulators
   b += (size_t)max_n * 4 * 4;   // xs, d, xo, hd
   b += (size_t)max_n * 16;      // V = {dg, g, pending xi, pending hd}
   b += (size_t)max_n * 8;       // per-lane coefficients of the pending update
@@ -827,10 +799,8 @@ __global__ void __launch_bounds__(NT, (HM == 1 && NT <= 256) ? 3 : 1) k_minimize
       TV = global_view(B, m);
     }
 
-    for (int i = tid; i < n; i += NT) {
-      xs[i] = P.x_in[off + i];
-      gacc[i] = 0ull;
-    }
+    for (int i = tid; i < n; i += NT) xs[i] = P.x_in[off + i];
+    for (int i = tid; i < n; i += NT) gacc[i] = 0ull;
     float psc = 1.0f;
     auto init_h = [&]() {
       if (ALGO != MMFFHIP_BFGS) return;
@@ -1098,6 +1068,34 @@ __global__ void __launch_bounds__(NT, (HM == 1 && NT <= 256) ? 3 : 1) k_minimize
           int lq = 0;
           while (lq < 3 && (nq << (lq + 1)) <= NT) lq++;
           const int lpr = 1 << lq, seg = (n + lpr - 1) >> lq;
+          // dot products of the pending updates' {s, h} with g, taken once per iteration
+          // (FP64 block sums) when the pass only reads H
+          float pdot[2][NPM];
+#pragma unroll
+          for (int k = 0; k < NPM; k++) pdot[0][k] = pdot[1][k] = 0.f;
+          if (npend > 0 && npend < NPM) {
+            double pv[2 * NPM];
+#pragma unroll
+            for (int k = 0; k < 2 * NPM; k++) pv[k] = 0.0;
+            for (int i = tid; i < n; i += NT) {
+              const double g = VG[i].y;
+#pragma unroll
+              for (int k = 0; k < NPM; k++) {
+                if (k < npend) {
+                  const float2 w = sh_slot(k)[i];
+                  pv[2 * k] += (double)w.x * g;
+                  pv[2 * k + 1] += (double)w.y * g;
+                }
+              }
+            }
+            R.template run<2 * NPM, 0>(pv);
+            // the sums are the same on every lane: keep them in scalar registers
+#pragma unroll
+            for (int k = 0; k < NPM; k++) {
+              pdot[0][k] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int((float)pv[2 * k])));
+              pdot[1][k] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int((float)pv[2 * k + 1])));
+            }
+          }
           for (int r = tid; r < (nq << lq); r += NT) {
             const int tq = r >> lq, q = r & (lpr - 1);
             const int j0 = q * seg, j1 = min(n, j0 + seg);
@@ -1117,33 +1115,21 @@ __global__ void __launch_bounds__(NT, (HM == 1 && NT <= 256) ? 3 : 1) k_minimize
             }
             float a2[4] = {0.f, 0.f, 0.f, 0.f};
             float4* base = (float4*)H + tq;  // row j of the quad at base[j * nq]
-            float lr[2][NPM];
-#pragma unroll
-            for (int k = 0; k < NPM; k++) lr[0][k] = lr[1][k] = 0.f;
-            hquad_dispatch<HQ, NPM>(npend, base, nq, j0, j1, VG, S, cx, cy, a2, lr);
-            // the pending updates' part of H g when they were not added element by element:
-            // sum the dot products over the quad's lanes, then y_t += alpha_t (s.g) + beta_t (h.g)
-            if (npend > 0 && npend < NPM) {
-#pragma unroll
-              for (int k = 0; k < NPM; k++)
-#pragma unroll
-                for (int c = 0; c < 2; c++) {
-                  float x = lr[c][k];
-                  if (lq >= 1) x += dpp_f<DPP_QUAD_XOR1>(x);
-                  if (lq >= 2) x += dpp_f<DPP_QUAD_XOR2>(x);
-                  if (lq >= 3) x += dpp_f<DPP_ROW_HALF_MIRROR>(x);
-                  lr[c][k] = x;
-                }
-            }
+            const bool flush = npend == NPM;
+            if (flush)
+              hquad_rows<HQ, true, NPM>(base, nq, j0, j1, VG, S, cx, cy, a2);
+            else
+              hquad_rows<HQ, false, NPM>(base, nq, j0, j1, VG, S, cx, cy, a2);
 #pragma unroll
             for (int u = 0; u < 4; u++) {
               float s2 = a2[u];
               if (lq >= 1) s2 += dpp_f<DPP_QUAD_XOR1>(s2);
               if (lq >= 2) s2 += dpp_f<DPP_QUAD_XOR2>(s2);
               if (lq >= 3) s2 += dpp_f<DPP_ROW_HALF_MIRROR>(s2);
-              if (npend > 0 && npend < NPM) {
+              // the pending updates' part of H g: y_t += alpha_t (s.g) + beta_t (h.g)
+              if (!flush && npend > 0) {
 #pragma unroll
-                for (int k = 0; k < NPM; k++) s2 = fmaf(cy[k][u], lr[1][k], fmaf(cx[k][u], lr[0][k], s2));
+                for (int k = 0; k < NPM; k++) s2 = fmaf(cy[k][u], pdot[1][k], fmaf(cx[k][u], pdot[0][k], s2));
               }
               const int t = 4 * tq + u;
               if (q == 0 && t < n) {
@@ -1641,7 +1627,9 @@ int check_batch(const mmffhip_batch* b) {
   return MMFFHIP_OK;
 }
 
-size_t eg_lds(int nt, int max_atoms) { return 16 * (nt / 64) * 8 + (size_t)3 * max_atoms * (8 + 4) + 16; }
+size_t eg_lds(int nt, int max_atoms) {
+  return 16 * (nt / 64) * 8 + (size_t)3 * max_atoms * (8 + 4) + 16;
+}
 
 template <int FAM>
 int launch_eg(const KBatch& kb, const float* x, double* e, double* et, float* g, uint32_t mask, int diel,
@@ -2026,7 +2014,7 @@ int mmffhip_multimol_run_ex(const mmffhip_batch* b, const float* x_in, float* x_
   KBatch kb = to_kbatch(b);
   constexpr int NT = 256;
   const int mx = 3 * b->max_atoms;
-  const size_t lds_init = 16 * (NT / 64) * 8 + (size_t)mx * 12 + 16;
+  const size_t lds_init = 16 * (NT / 64) * 8 + (size_t)mx * (8 + 4) + 16;
   const size_t lds_chunk = 16 * (NT / 64) * 8 + (size_t)mx * (8 + 16) + 8 * (size_t)lbfgs_m + 32;
   if ((int)lds_chunk > dev_info().lds) return fail(MMFFHIP_ERR_SIZE, "molecule too large for one workgroup");
   const int nfl = n_main + n_pol;

@@ -57,11 +57,10 @@ def restraints(params, z: np.ndarray):
     for (i, j), dd in tgt.items():
         ri.append(i), rj.append(j), d.append(dd), kind.append(0)
     iu, ju = np.triu_indices(n, 1)
-    mask = np.ones(len(iu), dtype=bool)
     if tgt:
-        keys = set(tgt)
-        mask = np.array([(a, b) not in keys for a, b in zip(iu.tolist(), ju.tolist())], dtype=bool)
-    iu, ju = iu[mask], ju[mask]
+        keys = np.array([a * n + b for a, b in tgt], dtype=np.int64)
+        mask = ~np.isin(iu.astype(np.int64) * n + ju, keys)
+        iu, ju = iu[mask], ju[mask]
     hh = heavy[iu].astype(int) + heavy[ju].astype(int)
     lb = np.where(hh == 2, 3.0, np.where(hh == 1, 2.55, 2.2))
     ri = np.concatenate([np.array(ri, np.int32), iu.astype(np.int32)])
@@ -99,4 +98,57 @@ def embed(params_list: Sequence, z_list: Sequence[np.ndarray], conf_counts: Sequ
             n_threads = os.cpu_count() or 1
     _lib().mmff_embed_batch(mols, conf_mol.ctypes.data, off.ctypes.data, sd.ctypes.data, len(conf_mol),
                             float(noise), out.ctypes.data, int(n_threads))
+    return out
+
+
+def pack_restraints(params_list: Sequence, z_list: Sequence[np.ndarray]):
+    """Ragged restraint tables of a batch for the device embedder: (mol_desc int32 [M, 4] =
+    {n_atoms, offset, count, 0}, restr_ij uint32 = i | j << 16, restr_d float32 = d0 for an
+    equality spring or -d_min for a lower-bound wall)."""
+    desc = np.zeros((len(params_list), 4), np.int32)
+    ij, dd = [], []
+    off = 0
+    for k, (p, z) in enumerate(zip(params_list, z_list)):
+        ri, rj, d, kind = restraints(p, np.asarray(z))
+        desc[k] = (p.n_atoms, off, len(ri), 0)
+        ij.append((ri.astype(np.uint32) & 0xFFFF) | ((rj.astype(np.uint32) & 0xFFFF) << 16))
+        dd.append(np.where(kind == 0, d, -d).astype(np.float32))
+        off += len(ri)
+    cat = lambda parts, dt: np.concatenate(parts).astype(dt) if parts else np.zeros(0, dt)  # noqa: E731
+    return desc, cat(ij, np.uint32), cat(dd, np.float32)
+
+
+def embed_device(params_list: Sequence, z_list: Sequence[np.ndarray], conf_counts: Sequence[int],
+                 seeds: Sequence[int], noise: float = 0.1, device: str = "cuda", stream=None):
+    """The same seeded restraint embedding as ``embed``, run on the GPU (one workgroup per
+    conformer, FP64, csrc/embed.hip). Returns the flat float32 coordinates as a device tensor,
+    ready for the minimizer without a host round trip. Results agree with ``embed`` in
+    distribution, not bit for bit (device FP64 math and fixed-point gradient sums)."""
+    from . import _lib
+    torch = _lib.require_gpu()
+    L = _lib.load()
+    counts = np.asarray(conf_counts, np.int64)
+    n_atoms = np.array([p.n_atoms for p in params_list], np.int64)
+    conf_mol = np.repeat(np.arange(len(params_list), dtype=np.int32), counts)
+    n3 = 3 * n_atoms[conf_mol]
+    off = np.zeros(len(conf_mol), np.int64)
+    if len(off) > 1:
+        off[1:] = np.cumsum(n3)[:-1]
+    total = int(n3.sum())
+    if total >= 2 ** 31:
+        raise ValueError("batch exceeds 2^31 coordinates")
+    sd = np.ascontiguousarray(np.asarray(seeds, dtype=np.uint64))
+    assert len(sd) == len(conf_mol)
+    desc, ij, dd = pack_restraints(params_list, z_list)
+    out = torch.empty(total, dtype=torch.float32, device=device)
+    if len(conf_mol) == 0:
+        return out
+    t = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(device)  # noqa: E731
+    bufs = [t(desc.reshape(-1)), t(ij.view(np.int32) if len(ij) else np.zeros(1, np.int32)),
+            t(dd if len(dd) else np.zeros(1, np.float32)), t(conf_mol), t(off.astype(np.int32)),
+            t(sd.view(np.int64))]
+    _lib.check(L.mmffhip_embed(*(int(b.data_ptr()) for b in bufs), len(conf_mol), int(n_atoms.max()), float(noise),
+                               int(out.data_ptr()), _lib.stream_ptr(stream)))
+    # the staging tensors must outlive the asynchronous launch
+    out._embed_inputs = bufs
     return out

@@ -26,16 +26,20 @@ def mmff_minimize_nk(mmff_params_list: List, conf_counts: List[int], positions_3
                      ls_grad: bool = False, return_info: bool = False):
     """Run MMFF94 optimization entirely on the GPU.
 
+    ``positions_3d`` may also be a float32 CUDA tensor (used in place on the device).
     Returns ``(out_pos float32 [sum 3N], energies float32 [C], converged bool [C])``; with
     ``return_info=True`` a fourth element holds FP64 energies, raw statuses and per-conformer
     counters {iterations, gradient evaluations, energy evaluations}.
     """
     torch = _lib.require_gpu()
-    pos = np.ascontiguousarray(np.asarray(positions_3d, dtype=np.float32).reshape(-1))
     db = device_batch(mmff_params_list, conf_counts)
-    if pos.size != db.host.total_coords:
-        raise ValueError(f"positions_3d has {pos.size} values, the batch needs {db.host.total_coords}")
-    x = torch.from_numpy(pos).cuda()
+    if isinstance(positions_3d, torch.Tensor) and positions_3d.is_cuda:
+        # start geometries already on the device (e.g. from embed_device): no host round trip
+        x = positions_3d.reshape(-1).to(torch.float32).contiguous()
+    else:
+        x = torch.from_numpy(np.ascontiguousarray(np.asarray(positions_3d, dtype=np.float32).reshape(-1))).cuda()
+    if x.numel() != db.host.total_coords:
+        raise ValueError(f"positions_3d has {x.numel()} values, the batch needs {db.host.total_coords}")
     r = NkRunner(db, max_iters=max_iters, grad_tol=grad_tol, use_lbfgs=use_lbfgs, lbfgs_m=lbfgs_m, diel=diel,
                  hess_place=hess_place, block=block, ls_grad=ls_grad)
     res = r.run(x)

@@ -10,8 +10,11 @@ If parameter extraction fails for a chunk, MMFF is skipped and the chunk is repo
 unconverged (:312-316, :341).
 
 The DG -> 4D -> 3D -> ETK embedding stages are outside this package's scope; start
-geometries come from the package's seeded embedder (no noise), and without MMFF the
-reported energy is NaN (there is no DG/ETK energy to report) with converged = True.
+geometries come from the package's seeded restraint embedder, run on the GPU
+(``embed_device``, no noise), and stay there for the MMFF stage: only each molecule's first
+conformer is copied to the host, for the 100 A non-bonded filter of the parameter
+extraction (:296-305). Without MMFF the reported energy is NaN (there is no DG/ETK energy to
+report) with converged = True.
 """
 from __future__ import annotations
 
@@ -21,7 +24,7 @@ from typing import List, Optional, Sequence
 
 import numpy as np
 
-from .embed import embed
+from .embed import embed_device
 from .mmff_params import extract_mmff_params
 from .mol import mol_from_smiles
 
@@ -127,27 +130,37 @@ def generate_conformers_nk(smiles_list: Sequence[str], n_confs_per_mol=10, *, ma
             kk[mol_idx] += b - a
         counts = [kk[m] for m in order]
         seeds = np.arange(sum(counts), dtype=np.uint64) + np.uint64(seed + 10000 * ci)
-        pos3 = embed([embed_params[m] for m in order], [mols[m].atomic_numbers() for m in order], counts,
-                     seeds=seeds, noise=0.0)
+        xdev = embed_device([embed_params[m] for m in order], [mols[m].atomic_numbers() for m in order], counts,
+                            seeds=seeds, noise=0.0)
         energies = np.full(sum(counts), np.nan, np.float64)
         conv = np.ones(sum(counts), bool)
+        pos3 = None
         if run_mmff:
+            import torch
             conv[:] = False
             plist = []
-            off = 0
+            # each molecule's first conformer, gathered on the device and copied in one transfer
+            firsts, off = [], 0
+            for m_idx, c in zip(order, counts):
+                firsts.append(np.arange(off, off + 3 * n_atoms[m_idx]))
+                off += 3 * n_atoms[m_idx] * c
+            first_x = xdev[torch.from_numpy(np.concatenate(firsts)).to(xdev.device)].cpu().numpy()
             try:
-                for m_idx, c in zip(order, counts):
+                off = 0
+                for m_idx in order:
                     n = n_atoms[m_idx]
-                    mols[m_idx].set_conformers(pos3[off:off + 3 * n].reshape(1, n, 3))
+                    mols[m_idx].set_conformers(first_x[off:off + 3 * n].reshape(1, n, 3))
                     plist.append(extract_mmff_params(mols[m_idx], mmff_variant=mmff_variant))
-                    off += 3 * n * c
+                    off += 3 * n
             except Exception:
                 plist = None
             if plist is not None:
                 iters = mmff_max_iters if mmff_max_iters > 0 else _auto_iters(max(n_atoms[m] for m in order), 200, 15.0)
-                pos3, e32, conv, info = mmff_minimize_nk(plist, counts, pos3, max_iters=iters,
+                pos3, e32, conv, info = mmff_minimize_nk(plist, counts, xdev, max_iters=iters,
                                                          use_lbfgs=bool(mmff_use_lbfgs), return_info=True)
                 energies = info["energy_f64"]
+        if pos3 is None:
+            pos3 = xdev.cpu().numpy()
         off = 0
         c = 0
         for m_idx, cnt in zip(order, counts):

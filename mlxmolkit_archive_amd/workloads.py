@@ -62,14 +62,11 @@ def _typed(args):
 
 
 def _pool_size(n_mols: int) -> int:
-    """Worker processes for typing: only for large batches, and only while no GPU context
-    exists in this process (the workers are forked)."""
+    """Worker processes for typing: only for large batches. The workers are started fresh
+    (spawned, not forked), so a GPU runtime already initialised in this process, by torch or
+    by a profiler, is never copied into them."""
     import os
-    import sys
     if n_mols < 2000:
-        return 1
-    torch = sys.modules.get("torch")
-    if torch is not None and torch.cuda.is_initialized():
         return 1
     try:
         cpus = len(os.sched_getaffinity(0))
@@ -80,7 +77,10 @@ def _pool_size(n_mols: int) -> int:
 
 def build(name: str, n_mols: Optional[int] = None, n_confs: Optional[int] = None,
           seed: int = molgen.SEED_BASE, noise: float = 0.1, variant: str = "MMFF94",
-          n_threads: int = 0) -> Workload:
+          n_threads: int = 0, embedder: str = "host") -> Workload:
+    """``embedder``: "host" (csrc/embed.c, threaded) or "device" (the same method on the GPU,
+    csrc/embed.hip; the start geometries then differ from the host's at the level of
+    floating-point rounding amplified by the relaxation, not in distribution)."""
     fam, M, K = SPECS[name]
     M = M if n_mols is None else n_mols
     K = K if n_confs is None else n_confs
@@ -89,7 +89,7 @@ def build(name: str, n_mols: Optional[int] = None, n_confs: Optional[int] = None
     if procs > 1:
         import multiprocessing as mp
         chunks = [mols[i::procs] for i in range(procs)]
-        with mp.get_context("fork").Pool(procs) as pool:
+        with mp.get_context("spawn").Pool(procs) as pool:
             parts = pool.map(_typed, [(c, variant) for c in chunks])
         params = [None] * M
         for i, part in enumerate(parts):
@@ -101,6 +101,12 @@ def build(name: str, n_mols: Optional[int] = None, n_confs: Optional[int] = None
     seeds = (np.uint64(seed) * np.uint64(1000003) + np.uint64(molgen._FAMILY_ID[fam]) * np.uint64(7919) * np.uint64(
         10 ** 9) + ids)
     z = [m.atomic_numbers() for m in mols]
+    if embedder == "device":
+        from .embed import embed_device
+        x = embed_device(params, z, counts, seeds=seeds, noise=noise).cpu().numpy()
+        return Workload(name=name, mols=mols, params=params, conf_counts=counts, x=x)
+    if embedder != "host":
+        raise ValueError(f"embedder must be 'host' or 'device', not {embedder!r}")
     # large batches are embedded in slices of molecules, with a progress line per slice
     step = M if M * K <= 20000 else max(1, 20000 // K)
     parts = []

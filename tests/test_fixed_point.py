@@ -1,8 +1,8 @@
 """The gradient accumulators' FP32 -> 64-bit fixed-point conversion (csrc/mmff_hip.hip to_fx),
 restated with numpy's IEEE float64 and checked against its definition: round(v * 2^28) with
 ties to even, exact negation symmetry (so +g and -g of a pair term cancel exactly), and an
-exact, order-independent sum. The device code performs the same IEEE operations (FP32 ->
-FP64 conversion, one FP64 add in round-to-nearest-even, a 64-bit integer subtract)."""
+exact, order-independent sum. The device code performs the same IEEE operations (a clamp,
+FP32 -> FP64 conversion, one FP64 add in round-to-nearest-even, a 64-bit integer subtract)."""
 from __future__ import annotations
 
 import numpy as np
@@ -14,7 +14,8 @@ SCALE = 2.0 ** 28
 
 def to_fx(v: np.ndarray) -> np.ndarray:
     v = np.asarray(v, np.float32)
-    c = np.where(v == v, np.clip(v, -FX_CAP, FX_CAP), np.float32(0.0)).astype(np.float32)
+    # fmaxf / fminf return the non-NaN operand: NaN -> -FX_CAP
+    c = np.where(v == v, np.clip(v, -FX_CAP, FX_CAP), -FX_CAP).astype(np.float32)
     y = c.astype(np.float64) + FX_MAGIC
     return y.view(np.int64) - np.array([FX_MAGIC]).view(np.int64)[0]
 
@@ -40,7 +41,7 @@ def test_negation_is_exact_and_cap_and_nan():
     assert np.array_equal(to_fx(-v), -to_fx(v))
     big = np.float32([1e9, -1e9, np.inf, -np.inf])
     assert np.array_equal(to_fx(big), np.rint(np.float64([8388000.0, -8388000.0] * 2) * SCALE).astype(np.int64))
-    assert to_fx(np.float32([np.nan]))[0] == 0
+    assert to_fx(np.float32([np.nan]))[0] == -int(np.rint(8388000.0 * SCALE))
 
 
 def test_sum_is_exact_and_order_independent():

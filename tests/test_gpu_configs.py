@@ -142,6 +142,7 @@ def test_chain_lbfgs_m10_global_history_tracks_oracle(chain_sample):
     ob = oracle.OracleBatch(wl.params, wl.conf_counts)
     x64 = wl.x.astype(np.float64)
     assert (3 * wl.atoms_per_conf() > 192).all()
+    xp = x64 + np.random.default_rng(2).normal(0.0, 1e-6, x64.shape)
     for k in (1, 3, 10, 30):
         r = engine.NkRunner(db, max_iters=k, grad_tol=1e-4, use_lbfgs=True, lbfgs_m=10, ls_grad=True)
         assert [(l["opts"].block, l["opts"].hess_place) for l in r.launches] == [(256, _lib.HESS_GLOBAL)]
@@ -161,7 +162,12 @@ def test_chain_lbfgs_m10_global_history_tracks_oracle(chain_sample):
             assert dx < 1e-3, k
             assert (de <= 1e-3 + 1e-6 * sabs).all(), k
         else:
-            assert np.median(de) < 1e-2, k
+            # after 30 iterations nearby trajectories of these floppy chains have separated:
+            # the GPU stays as close to the oracle as the oracle to its own 1e-6 A restart
+            ep = oracle.minimize_nk(ob, xp, max_iters=k, grad_tol=1e-4, use_lbfgs=True, m_hist=10)[1]
+            spread = float(np.median(np.abs(ep - er)))
+            print(f"k={k}: median |dE| gpu vs oracle {np.median(de):.3e}, oracle vs its restart {spread:.3e}")
+            assert np.median(de) < max(1e-2, 3.0 * spread), k
 
 
 def test_chain_lbfgs_m10_reference_budget_matches_oracle(chain_sample):
