@@ -78,7 +78,8 @@ def main():
                          "the headline target is quoted on; medium: 1,000 x 10; chain: 2,000 x 8 (with --lbfgs)")
     ap.add_argument("--mols", type=int, default=None)
     ap.add_argument("--confs", type=int, default=None)
-    ap.add_argument("--max-iters", type=int, default=2000)
+    ap.add_argument("--max-iters", type=int, default=2000,
+                    help="iteration cap; 0 = the reference pipeline's budget 200 + 15 x (largest molecule's atoms)")
     ap.add_argument("--grad-tol", type=float, default=1e-4)
     ap.add_argument("--lbfgs", action="store_true")
     ap.add_argument("--lbfgs-m", type=int, default=10,
@@ -116,6 +117,9 @@ def main():
         return bench_multimol(args, wl, t_gen)
     log(f"workload {args.workload}: {len(wl.params)} molecules, {wl.n_confs} conformers, "
         f"atoms {wl.atoms_per_conf().min()}-{wl.atoms_per_conf().max()} (built in {t_gen:.1f}s)")
+    if args.max_iters <= 0:  # conformer_pipeline_v2.py:123-126,169-170
+        args.max_iters = 200 + 15 * int(wl.atoms_per_conf().max())
+        log(f"iteration cap from the pipeline rule: {args.max_iters}")
 
     pc_info, pc = None, None
     if args.precond:

@@ -508,27 +508,36 @@ __device__ __forceinline__ void hquad_dot(const float4& h, float g, float (&a)[4
   a[3] = fmaf(h.w, g, a[3]);
 }
 
-// Rows [j, j1) of a column quad of the full Hessian (row r at base[r * nq]): accumulate the
-// stored rows' part of H g into a. With FLUSH, all NPM pending rank-2 updates (update k: row
-// vectors {s, h} in S[k], column coefficients cx[k], cy[k]) are first added to each element,
-// oldest first, and the rows are written back; without it the rows are only read (the
-// pending updates enter H g through dot products taken once per iteration). HQ rows are
-// loaded together.
-template <int HQ, bool FLUSH, int NPM>
+// Rows [j, j1) of a column quad of the full Hessian (row r at base[r * nq]): accumulate
+// H g into a, with H the stored rows plus the NU oldest pending rank-2 updates (update k:
+// row vectors {s, h} in S[k], column coefficients cx[k], cy[k]). When all NPM slots are
+// pending the updates are added to each element, oldest first, and the rows are written
+// back. Otherwise the stored rows are only read, and the updates' part of H g is left as two
+// dot products per update over these rows in lr (s.g, h.g), which the caller combines with
+// the column coefficients after summing the quad's lanes. HQ rows are loaded together.
+template <int HQ, int NU, int NPM>
 __device__ __forceinline__ void hquad_rows(float4* base, int nq, int j, int j1, const float2* VG,
                                            const float2* const (&S)[NPM], const float (&cx)[NPM][4],
-                                           const float (&cy)[NPM][4], float (&a)[4]) {
+                                           const float (&cy)[NPM][4], float (&a)[4], float (&lr)[2][NPM]) {
+  constexpr bool FLUSH = NU == NPM;
   for (; j + HQ - 1 < j1; j += HQ) {
     float4 h[HQ];
 #pragma unroll
     for (int u = 0; u < HQ; u++) h[u] = base[(size_t)(j + u) * nq];
 #pragma unroll
     for (int u = 0; u < HQ; u++) {
-      if (FLUSH) {
+      const float g = VG[j + u].y;
 #pragma unroll
-        for (int k = 0; k < NPM; k++) hquad_upd(h[u], S[k][j + u], cx[k], cy[k]);
+      for (int k = 0; k < NU; k++) {
+        const float2 w = S[k][j + u];
+        if (FLUSH) {
+          hquad_upd(h[u], w, cx[k], cy[k]);
+        } else {
+          lr[0][k] = fmaf(w.x, g, lr[0][k]);
+          lr[1][k] = fmaf(w.y, g, lr[1][k]);
+        }
       }
-      hquad_dot(h[u], VG[j + u].y, a);
+      hquad_dot(h[u], g, a);
     }
     if (FLUSH) {
 #pragma unroll
@@ -537,12 +546,34 @@ __device__ __forceinline__ void hquad_rows(float4* base, int nq, int j, int j1, 
   }
   for (; j < j1; j++) {
     float4 h = base[(size_t)j * nq];
-    if (FLUSH) {
+    const float g = VG[j].y;
 #pragma unroll
-      for (int k = 0; k < NPM; k++) hquad_upd(h, S[k][j], cx[k], cy[k]);
+    for (int k = 0; k < NU; k++) {
+      const float2 w = S[k][j];
+      if (FLUSH) {
+        hquad_upd(h, w, cx[k], cy[k]);
+      } else {
+        lr[0][k] = fmaf(w.x, g, lr[0][k]);
+        lr[1][k] = fmaf(w.y, g, lr[1][k]);
+      }
     }
-    hquad_dot(h, VG[j].y, a);
+    hquad_dot(h, g, a);
     if (FLUSH) base[(size_t)j * nq] = h;
+  }
+}
+
+// hquad_rows for the run-time count of pending updates (0..NPM)
+template <int HQ, int NPM, int NU = NPM>
+__device__ __forceinline__ void hquad_dispatch(int npend, float4* base, int nq, int j, int j1, const float2* VG,
+                                               const float2* const (&S)[NPM], const float (&cx)[NPM][4],
+                                               const float (&cy)[NPM][4], float (&a)[4], float (&lr)[2][NPM]) {
+  if constexpr (NU == 0) {
+    hquad_rows<HQ, 0, NPM>(base, nq, j, j1, VG, S, cx, cy, a, lr);
+  } else {
+    if (npend == NU)
+      hquad_rows<HQ, NU, NPM>(base, nq, j, j1, VG, S, cx, cy, a, lr);
+    else
+      hquad_dispatch<HQ, NPM, NU - 1>(npend, base, nq, j, j1, VG, S, cx, cy, a, lr);
   }
 }
 
@@ -1068,34 +1099,6 @@ __global__ void __launch_bounds__(NT, (HM == 1 && NT <= 256) ? 3 : 1) k_minimize
           int lq = 0;
           while (lq < 3 && (nq << (lq + 1)) <= NT) lq++;
           const int lpr = 1 << lq, seg = (n + lpr - 1) >> lq;
-          // dot products of the pending updates' {s, h} with g, taken once per iteration
-          // (FP64 block sums) when the pass only reads H
-          float pdot[2][NPM];
-#pragma unroll
-          for (int k = 0; k < NPM; k++) pdot[0][k] = pdot[1][k] = 0.f;
-          if (npend > 0 && npend < NPM) {
-            double pv[2 * NPM];
-#pragma unroll
-            for (int k = 0; k < 2 * NPM; k++) pv[k] = 0.0;
-            for (int i = tid; i < n; i += NT) {
-              const double g = VG[i].y;
-#pragma unroll
-              for (int k = 0; k < NPM; k++) {
-                if (k < npend) {
-                  const float2 w = sh_slot(k)[i];
-                  pv[2 * k] += (double)w.x * g;
-                  pv[2 * k + 1] += (double)w.y * g;
-                }
-              }
-            }
-            R.template run<2 * NPM, 0>(pv);
-            // the sums are the same on every lane: keep them in scalar registers
-#pragma unroll
-            for (int k = 0; k < NPM; k++) {
-              pdot[0][k] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int((float)pv[2 * k])));
-              pdot[1][k] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int((float)pv[2 * k + 1])));
-            }
-          }
           for (int r = tid; r < (nq << lq); r += NT) {
             const int tq = r >> lq, q = r & (lpr - 1);
             const int j0 = q * seg, j1 = min(n, j0 + seg);
@@ -1115,21 +1118,33 @@ __global__ void __launch_bounds__(NT, (HM == 1 && NT <= 256) ? 3 : 1) k_minimize
             }
             float a2[4] = {0.f, 0.f, 0.f, 0.f};
             float4* base = (float4*)H + tq;  // row j of the quad at base[j * nq]
-            const bool flush = npend == NPM;
-            if (flush)
-              hquad_rows<HQ, true, NPM>(base, nq, j0, j1, VG, S, cx, cy, a2);
-            else
-              hquad_rows<HQ, false, NPM>(base, nq, j0, j1, VG, S, cx, cy, a2);
+            float lr[2][NPM];
+#pragma unroll
+            for (int k = 0; k < NPM; k++) lr[0][k] = lr[1][k] = 0.f;
+            hquad_dispatch<HQ, NPM>(npend, base, nq, j0, j1, VG, S, cx, cy, a2, lr);
+            // the pending updates' part of H g when they were not added element by element:
+            // sum the dot products over the quad's lanes, then y_t += alpha_t (s.g) + beta_t (h.g)
+            if (npend > 0 && npend < NPM) {
+#pragma unroll
+              for (int k = 0; k < NPM; k++)
+#pragma unroll
+                for (int c = 0; c < 2; c++) {
+                  float x = lr[c][k];
+                  if (lq >= 1) x += dpp_f<DPP_QUAD_XOR1>(x);
+                  if (lq >= 2) x += dpp_f<DPP_QUAD_XOR2>(x);
+                  if (lq >= 3) x += dpp_f<DPP_ROW_HALF_MIRROR>(x);
+                  lr[c][k] = x;
+                }
+            }
 #pragma unroll
             for (int u = 0; u < 4; u++) {
               float s2 = a2[u];
               if (lq >= 1) s2 += dpp_f<DPP_QUAD_XOR1>(s2);
               if (lq >= 2) s2 += dpp_f<DPP_QUAD_XOR2>(s2);
               if (lq >= 3) s2 += dpp_f<DPP_ROW_HALF_MIRROR>(s2);
-              // the pending updates' part of H g: y_t += alpha_t (s.g) + beta_t (h.g)
-              if (!flush && npend > 0) {
+              if (npend > 0 && npend < NPM) {
 #pragma unroll
-                for (int k = 0; k < NPM; k++) s2 = fmaf(cy[k][u], pdot[1][k], fmaf(cx[k][u], pdot[0][k], s2));
+                for (int k = 0; k < NPM; k++) s2 = fmaf(cy[k][u], lr[1][k], fmaf(cx[k][u], lr[0][k], s2));
               }
               const int t = 4 * tq + u;
               if (q == 0 && t < n) {

@@ -104,7 +104,15 @@ def test_ragged_every_default_bucket_matches_oracle(ragged_sample):
     st = res.status.cpu().numpy()
     e_rep = res.energy.cpu().numpy()
     n3 = 3 * wl.atoms_per_conf()
-    assert np.abs(e_rep - e_gpu).max() < 1e-3  # reported FP64 energy = FP64 re-evaluation
+    # the reported energy (FP32 terms, FP64 sum) is the FP64 re-evaluation up to FP32 term
+    # rounding, which scales with the terms' magnitudes
+    xg = res.x.cpu().numpy().astype(np.float64)
+    sabs = np.array([oracle.terms_abs(wl.params[ob.conf_mol[c]],
+                                      xg[int(ob.conf_off[c]):int(ob.conf_off[c]) + 3 * wl.params[ob.conf_mol[c]].n_atoms]
+                                      ).sum() for c in range(ob.n_confs)])
+    print(f"\nreported vs FP64 re-evaluated energy: max {np.abs(e_rep - e_gpu).max():.2e}, "
+          f"max relative to sum|terms| {np.max(np.abs(e_rep - e_gpu) / sabs):.2e}")
+    assert (np.abs(e_rep - e_gpu) <= 1e-3 + 2e-6 * sabs).all()
     d = e_gpu - e_ref
     print(f"\noracle self-agreement (1e-6 A restart): {floor:.3f}")
     for name, lo, hi in (("LDS n<=96 x128", 0, 96), ("global n<=360 x256", 96, 360), ("global n>360 x512", 360, 1 << 20)):
