@@ -58,12 +58,13 @@ def algorithmic_cost(wl, hb, counters, dense=True, m_hist=8):
     tb = np.repeat([canonical_term_bytes(p) for p in wl.params], wl.conf_counts).astype(np.float64)
     n = 3.0 * wl.atoms_per_conf()
     it, gr, ev = (counters[:, k].astype(np.float64) for k in range(3))
-    per_it = 12 * n * n + 40 * n if dense else 16 * m_hist * n + 48 * n
+    dense = np.asarray(dense, bool)  # one flag for all, or per conformer (per-size algorithm)
+    per_it = np.where(dense, 12 * n * n + 40 * n, 16 * m_hist * n + 48 * n)
     B = it * per_it + gr * (tb + 8 * n) + ev * (tb + 4 * n)
     cnt = _mol_counts(hb)
     fe = sum(cnt[k] * TERM_FLOPS[k][0] for k in TERM_FLOPS)[hb.conf_mol]
     feg = sum(cnt[k] * TERM_FLOPS[k][1] for k in TERM_FLOPS)[hb.conf_mol]
-    f_it = 10 * n * n + 30 * n if dense else 8 * m_hist * n + 30 * n
+    f_it = np.where(dense, 10 * n * n + 30 * n, 8 * m_hist * n + 30 * n)
     F = it * f_it + gr * feg + ev * fe
     return B, F
 
@@ -84,6 +85,9 @@ def main():
     ap.add_argument("--lbfgs", action="store_true")
     ap.add_argument("--lbfgs-m", type=int, default=10,
                     help="L-BFGS history length with --lbfgs (the chain configuration uses m = 10)")
+    ap.add_argument("--lbfgs-above", type=int, default=0,
+                    help="choose the method per conformer: dense BFGS up to n = 3N of this value, L-BFGS "
+                         "(m = --lbfgs-m, 128 threads, history in global memory) above; 0 = off")
     ap.add_argument("--sample", type=int, default=2048, help="conformers checked by the FP64 oracle")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--oracle-restart", action="store_true",
@@ -141,8 +145,21 @@ def main():
     t0 = time.perf_counter()
     db = engine.device_batch(wl.params, wl.conf_counts)
     x_dev = torch.from_numpy(wl.x).cuda()
-    runner = engine.NkRunner(db, max_iters=args.max_iters, grad_tol=args.grad_tol, use_lbfgs=args.lbfgs,
-                             lbfgs_m=args.lbfgs_m, ls_grad=True, precond=pc)
+    buckets = None
+    if args.lbfgs_above > 0:
+        assert not args.lbfgs and pc is None, "--lbfgs-above mixes methods; it excludes --lbfgs and --precond"
+        buckets = []
+        for b, blk, pl in engine.DEFAULT_BUCKETS:
+            buckets.append((min(b, args.lbfgs_above), blk, pl, "bfgs"))
+            if b >= args.lbfgs_above:
+                break
+        buckets = tuple(buckets) + ((1 << 20, 128, _lib.HESS_GLOBAL, "lbfgs"),)
+    runner = engine.NkRunner(db, buckets=buckets, max_iters=args.max_iters, grad_tol=args.grad_tol,
+                             use_lbfgs=args.lbfgs, lbfgs_m=args.lbfgs_m, ls_grad=True, precond=pc)
+    # per conformer: does it run dense BFGS?
+    dense_conf = np.zeros(wl.n_confs, bool)
+    for ln in runner.launches:
+        dense_conf[ln["order"].cpu().numpy()] = ln["algo"] == "bfgs"
     torch.cuda.synchronize()
     t_pack = time.perf_counter() - t0
     log(f"packed + uploaded in {t_pack:.1f}s; launches "
@@ -185,7 +202,7 @@ def main():
     e_gpu = res.energy.cpu().numpy()
     xg = res.x.cpu().numpy()
     n_status0 = int((st == 0).sum())
-    Bc, Fc = algorithmic_cost(wl, db.host, cnt, dense=not args.lbfgs, m_hist=args.lbfgs_m)
+    Bc, Fc = algorithmic_cost(wl, db.host, cnt, dense=dense_conf, m_hist=args.lbfgs_m)
     # the roofline is priced on the dominant launch: its conformers' algorithmic bytes and
     # flops over its own measured duration (the launches of a step run concurrently)
     launches = []
@@ -227,9 +244,28 @@ def main():
         if pc is not None:  # the oracle starts from the same learned blocks
             pcn = pc.cpu().numpy().astype(np.float64)
             h0s = np.concatenate([pcn[off[c] // 3:(off[c] + n3[c]) // 3] for c in pick])
-        _, e_ref, st_ref, cnt_ref = oracle.minimize_nk(ob, xs, max_iters=args.max_iters, grad_tol=args.grad_tol,
-                                                       use_lbfgs=args.lbfgs, m_hist=args.lbfgs_m, n_threads=threads,
-                                                       h0=h0s)
+        n3s = n3[pick]
+        offs = np.concatenate([[0], np.cumsum(n3s)])
+
+        def run_oracle(xa):
+            """The oracle on the sample, each conformer with the method the GPU ran it with."""
+            e_o, st_o, cnt_o = np.zeros(S), np.zeros(S, np.int32), np.zeros((S, 3), np.int32)
+            for dense in (True, False):
+                idx = np.nonzero(dense_conf[pick] == dense)[0]
+                if len(idx) == 0:
+                    continue
+                xk = np.concatenate([xa[offs[i]:offs[i + 1]] for i in idx])
+                obk = ob if len(idx) == S else oracle.OracleBatch([sub_params[i] for i in idx], [1] * len(idx))
+                h0k = None
+                if h0s is not None:
+                    h0k = np.concatenate([h0s[offs[i] // 3:offs[i + 1] // 3] for i in idx])
+                _, e_k, st_k, c_k = oracle.minimize_nk(obk, xk, max_iters=args.max_iters, grad_tol=args.grad_tol,
+                                                       use_lbfgs=not dense, m_hist=args.lbfgs_m, n_threads=threads,
+                                                       h0=h0k)
+                e_o[idx], st_o[idx], cnt_o[idx] = e_k, st_k, c_k
+            return e_o, st_o, cnt_o
+
+        e_ref, st_ref, cnt_ref = run_oracle(xs)
         t_cpu = time.perf_counter() - t0
         e_gpu_fp64, _, _ = oracle.energy_grad_batch(ob, xgs, family=oracle.FAMILY_TG, n_threads=threads)
         dEs = e_gpu_fp64 - e_ref
@@ -241,7 +277,8 @@ def main():
         cpu = {"value": round(float((st_ref == 0).sum() / t_cpu), 3), "unit": "converged conformers/s",
                "cores": int(threads), "kind": "port",
                "sample": f"{S} seeded conformers of the same workload, FP64 oracle "
-                         f"{'L-BFGS' if args.lbfgs else 'BFGS'} (same control flow), {t_cpu:.1f}s wall"}
+                         f"{'L-BFGS' if args.lbfgs else 'BFGS + L-BFGS per size' if args.lbfgs_above else 'BFGS'} "
+                         f"(same control flow as each conformer's kernel), {t_cpu:.1f}s wall"}
         sample_info = {"sample": S, "status0_in_sample": int(ok0.sum()), "oracle_status0": int((st_ref == 0).sum()),
                        "within_0.1_in_sample": int(within.sum()),
                        # misses among conformers both sides converged: did the GPU end lower or higher?
@@ -253,8 +290,7 @@ def main():
                        "max_reported_vs_fp64_dE": float(np.max(np.abs(e_gpu[pick] - e_gpu_fp64)))}
         if args.oracle_restart:
             xp = xs + np.random.default_rng(12345).normal(0.0, 1e-6, xs.shape)
-            _, e_p, st_p, _ = oracle.minimize_nk(ob, xp, max_iters=args.max_iters, grad_tol=args.grad_tol,
-                                                 use_lbfgs=args.lbfgs, m_hist=args.lbfgs_m, n_threads=threads, h0=h0s)
+            e_p, st_p, _ = run_oracle(xp)
             okp = (st_ref == 0) & (st_p == 0)
             sample_info["oracle_vs_restart_within_0.1"] = round(float(np.mean(np.abs(e_p - e_ref)[okp] <= 0.1)), 4)
         log(f"cpu_baseline {t_cpu:.1f}s; within 0.1 kcal/mol: {within.sum()}/{ok0.sum()} status-0 sampled "
@@ -264,7 +300,7 @@ def main():
     # FETCH_SIZE / WRITE_SIZE passes over the same command, scaled by the copy-kernel
     # calibration, read from the committed profile summary (a separate profiler run)
     traffic, traffic_src = None, None
-    default_shape = (args.mols is None and args.confs is None and not args.lbfgs and pc is None
+    default_shape = (args.mols is None and args.confs is None and not args.lbfgs and pc is None and not args.lbfgs_above
                      and args.max_iters == 2000 and args.grad_tol == 1e-4 and args.embedder == "device")
     prof = ROOT / "profiles" / f"k_minimize_nk_{args.workload}_traffic.json"
     if default_shape and prof.exists():
@@ -291,7 +327,9 @@ def main():
         "data": f"synthetic: seeded molecules, typed by the built-in MMFF94-style typer, embedded ({args.embedder} "
                 "restraint embedder) + N(0,0.1A) noise",
         "config": {"workload": f"{args.workload}: {len(wl.params)} molecules x {wl.conf_counts[0]} conformers",
-                   "algorithm": (f"L-BFGS m={args.lbfgs_m}" if args.lbfgs else "dense BFGS")
+                   "algorithm": (f"L-BFGS m={args.lbfgs_m}" if args.lbfgs else
+                                 f"dense BFGS up to n={args.lbfgs_above}, L-BFGS m={args.lbfgs_m} above"
+                                 if args.lbfgs_above else "dense BFGS")
                                 + (" from learned H0" if pc is not None else ""), "max_iters": args.max_iters,
                    "grad_tol": args.grad_tol, "atoms_min_max": [int(wl.atoms_per_conf().min()),
                                                                 int(wl.atoms_per_conf().max())],

@@ -123,6 +123,15 @@ int mmffhip_energy_grad(const mmffhip_batch *b, int family, int diel, const floa
 /* Workspace bytes mmffhip_minimize_nk needs for this batch and options. */
 int mmffhip_minimize_nk_workspace(const mmffhip_batch *b, const mmffhip_nk_opts *o, size_t *bytes);
 
+/* The launch mmffhip_minimize_nk would make for this batch and options: the persistent grid
+ * (resident workgroups, at most one per conformer), threads per workgroup, LDS bytes per
+ * workgroup and the Hessian placement chosen (MMFFHIP_HESS_LDS / _GLOBAL / _GLOBAL_PACKED).
+ * block, lds_bytes and hess_place may be NULL. No counterpart in the reference, which
+ * launches one threadgroup per conformer (mlxmolkit/mmff_minimize.py:310-311,340-341): callers
+ * use it to size their size buckets against the device's residency. */
+int mmffhip_minimize_nk_plan(const mmffhip_batch *b, const mmffhip_nk_opts *o, int32_t *grid, int32_t *block,
+                             int32_t *lds_bytes, int32_t *hess_place);
+
 /* In-kernel BFGS / L-BFGS minimization of every conformer (reference run_mmff semantics).
  * x_in / x_out may alias. order: conformer processing order (length n_confs, e.g. longest
  * first), NULL = natural order. counters (optional): 3 int32 per conformer

@@ -48,6 +48,7 @@ SIGNATURES = {
     "mmffhip_device_info": (C.c_int, [_vp, _vp, _vp]),
     "mmffhip_energy_grad": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp, C.c_uint32, _vp]),
     "mmffhip_minimize_nk_workspace": (C.c_int, [_vp, _vp, _vp]),
+    "mmffhip_minimize_nk_plan": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "mmffhip_minimize_nk": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_size_t, _vp]),
     "mmffhip_minimize_nk_noalloc": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "mmffhip_bb_fused": (C.c_int, [_vp, _vp, _vp, _vp, C.c_int32, C.c_float, C.c_float, _vp]),

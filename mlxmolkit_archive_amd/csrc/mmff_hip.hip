@@ -1681,7 +1681,7 @@ const void* nk_kernel_ptr() {
   X(0, 0, 0, 512) X(0, 0, 0, 640) X(0, 0, 0, 768) X(0, 0, 0, 1024)                                     \
   X(0, 1, 0, 64) X(0, 1, 0, 128) X(0, 1, 0, 192) X(0, 1, 0, 256) X(0, 1, 0, 320) X(0, 1, 0, 512)        \
   X(0, 1, 0, 640) X(0, 1, 0, 768) X(0, 1, 0, 1024)                                                     \
-  X(0, 1, 2, 128) X(0, 1, 2, 192) X(0, 1, 2, 256) X(0, 1, 2, 320) X(0, 1, 2, 512)                      \
+  X(0, 1, 2, 64) X(0, 1, 2, 128) X(0, 1, 2, 192) X(0, 1, 2, 256) X(0, 1, 2, 320) X(0, 1, 2, 512)       \
   X(0, 2, 0, 256) X(0, 2, 0, 512) X(0, 2, 0, 640) X(0, 2, 0, 1024)                                     \
   X(1, 0, 1, 64) X(1, 0, 1, 128) X(1, 0, 1, 256)                                                       \
   X(1, 0, 0, 128) X(1, 0, 0, 256) X(1, 0, 0, 512)                                                      \
@@ -1888,6 +1888,22 @@ int mmffhip_minimize_nk_workspace(const mmffhip_batch* b, const mmffhip_nk_opts*
   rc = make_nk_plan(b, o, pl);
   if (rc) return rc;
   *bytes = nk_ws_bytes(pl);
+  return MMFFHIP_OK;
+}
+
+int mmffhip_minimize_nk_plan(const mmffhip_batch* b, const mmffhip_nk_opts* o, int32_t* grid, int32_t* block,
+                             int32_t* lds_bytes, int32_t* hess_place) {
+  int rc = check_batch(b);
+  if (rc) return rc;
+  if (!grid) return fail(MMFFHIP_ERR_ARG, "grid is NULL");
+  NkPlan pl;
+  rc = make_nk_plan(b, o, pl);
+  if (rc) return rc;
+  *grid = b->n_confs == 0 ? 0 : pl.grid;
+  if (block) *block = pl.nt;
+  if (lds_bytes) *lds_bytes = (int32_t)pl.lds;
+  if (hess_place) *hess_place = pl.hm == 0 ? MMFFHIP_HESS_LDS : pl.hm == 2 ? MMFFHIP_HESS_GLOBAL_PACKED
+                                                                           : MMFFHIP_HESS_GLOBAL;
   return MMFFHIP_OK;
 }
 

@@ -65,6 +65,17 @@ def nk_options(max_iters=500, grad_tol=1e-4, use_lbfgs=False, lbfgs_m=8, diel=1,
     return o
 
 
+def resident_workgroups(desc, o) -> int:
+    """Workgroups the persistent minimize_nk launch keeps resident on the device for this
+    batch descriptor's molecules and these options (the grid of an unbounded batch)."""
+    L = _lib.load()
+    d = _lib.Batch.from_buffer_copy(desc)
+    d.n_confs = 1 << 30
+    g = C.c_int32()
+    _lib.check(L.mmffhip_minimize_nk_plan(C.byref(d), C.byref(o), C.byref(g), None, None, None))
+    return int(g.value)
+
+
 def table_lds_bytes(hb: HostBatch, pairs: bool = True) -> np.ndarray:
     """Per-molecule bytes of the LDS staging layout of its term tables (csrc table_lds_bytes);
     without ``pairs``, of its bonded tables only (what a global-Hessian launch stages)."""
@@ -75,33 +86,53 @@ def table_lds_bytes(hb: HostBatch, pairs: bool = True) -> np.ndarray:
 
 
 # Size buckets for the nk minimizer: (max 3N, threads per workgroup, Hessian placement).
-# Each bucket is its own persistent launch sized for its largest molecule. Small molecules
-# keep the packed inverse Hessian in LDS. Above n = 96 the full-storage Hessian in global
-# memory (L2 / Infinity Cache resident, read as coalesced 16-byte column quads) wins: its
-# workgroups need little LDS, so three 256-thread workgroups share a CU and one's memory
-# waits overlap the others' term evaluation, where an LDS-resident triangle above n = 96
-# holds a whole CU's LDS for one workgroup. Above n = 360 one 512-thread workgroup per CU
-# does better: three large Hessians per CU outgrow the Infinity Cache. Measured on MI355X:
-# medium workload (1,000 x 10, n <= 330) 173 ms per step against 305 ms for size-matched LDS
-# buckets; ragged workload (200k conformers, n <= 471) 4.89 s against 5.48 s with one global
-# bucket at 256 threads.
-DEFAULT_BUCKETS = ((96, 128, _lib.HESS_LDS), (360, 256, _lib.HESS_GLOBAL), (1 << 20, 512, _lib.HESS_GLOBAL))
-# L-BFGS keeps its (s, y) history in LDS for small molecules and in global memory above
-# n = 192 (256 threads): chain workload subset (250 x 8, n = 450-1200), 855 ms per step
-# against 1071 ms with LDS history up to n = 600.
-LBFGS_BUCKETS = ((192, 128, _lib.HESS_LDS), (1 << 20, 256, _lib.HESS_GLOBAL))
+# Each bucket is its own persistent launch sized for its largest molecule. Every default
+# bucket keeps a full-storage inverse Hessian in a global-memory slot per resident workgroup
+# (L2 / Infinity Cache resident, read as coalesced 16-byte column quads): those workgroups
+# need little LDS, so registers set the residency (three waves per SIMD up to 256 threads)
+# and one conformer's memory waits overlap the others' term evaluation. Small molecules run
+# one conformer per wave (64 threads, 12 conformers per CU), where a packed LDS triangle
+# holds 4-7 conformers per CU. Above n = 360 one 512-thread workgroup per CU does better:
+# three large Hessians per CU outgrow the Infinity Cache. Measured on MI355X, whole size
+# classes of the ragged workload (200k conformers), ms per class:
+#   n <= 96:        64 threads 56.5, 128 threads 66.2, LDS triangle at 128 threads 90.9
+#   96 < n <= 144:  64: 126.2, 128: 125.7, 192: 147.6, 256: 168.5
+#   144 < n <= 192: 64: 242.1, 128: 199.2, 192: 196.9, 256: 226.0
+#   192 < n <= 264: 128: 591.2, 192: 548.4, 256: 509.6
+#   264 < n <= 360: 128: 1841.5, 192: 1615.4, 256: 1487.7, 512: 1777.7
+#   n > 360:        256: 1986.3, 512: 1727.8
+DEFAULT_BUCKETS = ((96, 64, _lib.HESS_GLOBAL), (192, 128, _lib.HESS_GLOBAL), (360, 256, _lib.HESS_GLOBAL),
+                   (1 << 20, 512, _lib.HESS_GLOBAL))
+# L-BFGS keeps its (s, y) history in global memory: 128 threads up to n = 600, 256 above.
+# Whole classes, m = 10, maxIters 2000, ms per class: ragged n <= 192 (119,560 conformers)
+# 704 with global history against 1233 with the history in LDS; ragged 192 < n <= 600
+# (80,440): 128 threads 2313, 192: 2690, 256: 2676; chain n > 600 (10,928): 128: 5144,
+# 192: 3899, 256: 3149, 512: 4939. The chain workload's 5,072 conformers of n <= 600 fill
+# the 128-thread launch only three times over, so they fold into the 256-thread one.
+LBFGS_BUCKETS = ((600, 128, _lib.HESS_GLOBAL), (1 << 20, 256, _lib.HESS_GLOBAL))
 
 
 class NkRunner:
     """Pre-planned minimize_nk launches for one batch (one launch per size bucket).
 
-    ``buckets``: sequence of (max 3N, block, hess_place); conformers go to the first bucket
-    whose bound covers them (``autotune_buckets`` picks them by measurement). ``order``:
-    optional conformer order (default: longest first); conformers left out of it are not
-    minimized.
+    ``buckets``: sequence of (max 3N, block, hess_place) or (max 3N, block, hess_place, algo);
+    conformers go to the first bucket whose bound covers them (``autotune_buckets`` picks them
+    by measurement). ``algo`` ("bfgs" or "lbfgs") chooses the method per bucket, so dense BFGS
+    or L-BFGS is chosen per conformer by its size; without it every bucket runs the method of
+    ``use_lbfgs``. ``order``: optional conformer order (default: longest first); conformers left
+    out of it are not minimized.
+
+    ``min_fill``: a bucket whose conformers number fewer than ``min_fill`` times the workgroups
+    its persistent launch keeps resident (``mmffhip_minimize_nk_plan``) is folded into the next
+    bucket of the same method. Launches run one after another, and each ends with a tail in
+    which its last conformers leave most of the device idle; a bucket that fills the device
+    only a few times over loses more to its tail than its better shape gains (medium workload:
+    163.8 ms per step with a 128-thread launch for its 4,510 conformers of 96 < n <= 192,
+    against 149.7 ms with them in the 256-thread launch). 0 keeps every bucket.
     """
 
-    def __init__(self, db: DeviceBatch, order: Optional[np.ndarray] = None, buckets=None, **opts):
+    def __init__(self, db: DeviceBatch, order: Optional[np.ndarray] = None, buckets=None, min_fill: int = 8,
+                 **opts):
         torch = _lib.require_gpu()
         self.db = db
         self._precond = opts.get("precond")
@@ -122,24 +153,35 @@ class NkRunner:
         L = _lib.load()
         self.launches = []
         assigned = np.zeros(hb.n_confs, bool)
-        for bound, block, place in buckets:
+        algos = [bk[3] if len(bk) > 3 else ("lbfgs" if use_lbfgs else "bfgs") for bk in buckets]
+        for k, bk in enumerate(buckets):
+            bound, block, place = bk[:3]
+            algo = algos[k]
+            if algo not in ("bfgs", "lbfgs"):
+                raise ValueError(f"bucket algo must be 'bfgs' or 'lbfgs', not {algo!r}")
             sel = order[(n3[order] <= bound) & ~assigned[order]]
             if len(sel) == 0:
                 continue
-            assigned[sel] = True
             desc = _lib.Batch.from_buffer_copy(db.desc)
             desc.n_confs = len(sel)
             desc.max_atoms = int(hb.n_atoms[hb.conf_mol[sel]].max())
             hp = user_place or place
             per_mol = bbytes_mol if hp == _lib.HESS_GLOBAL else tbytes_mol
             tb = int(per_mol[hb.conf_mol[sel]].max()) if stage else 0
-            o = nk_options(block=user_block or block, hess_place=hp, table_lds_bytes=tb, **opts)
+            bopts = dict(opts, use_lbfgs=algo == "lbfgs")
+            o = nk_options(block=user_block or block, hess_place=hp, table_lds_bytes=tb, **bopts)
+            # too few to fill the device many times over: the next bucket (same method, larger
+            # bound) takes them
+            fold = min_fill > 0 and k + 1 < len(buckets) and algos[k + 1] == algo and buckets[k + 1][0] >= bound
+            if fold and len(sel) < min_fill * resident_workgroups(desc, o):
+                continue
+            assigned[sel] = True
             nbytes = C.c_size_t()
             _lib.check(L.mmffhip_minimize_nk_workspace(C.byref(desc), C.byref(o), C.byref(nbytes)))
             ws = torch.empty(int(nbytes.value), dtype=torch.uint8, device=dev)
             ordt = torch.from_numpy(np.ascontiguousarray(sel)).to(dev)
             self.launches.append(dict(desc=desc, opts=o, ws=ws, ws_bytes=int(nbytes.value), order=ordt,
-                                      n=len(sel), max_n=3 * desc.max_atoms, tbytes=tb))
+                                      n=len(sel), max_n=3 * desc.max_atoms, tbytes=tb, algo=algo))
         assert assigned[order].all(), "a conformer fits no bucket"
         C_ = db.n_confs
         self.e = torch.empty(C_, dtype=torch.float64, device=dev)
@@ -210,19 +252,22 @@ class NkRunner:
 # candidate (block, placement) pairs per size class for the autotuner; LDS candidates
 # whose plan does not fit are skipped
 TUNE_CLASSES = (
-    (96, ((128, _lib.HESS_LDS), (256, _lib.HESS_LDS), (128, _lib.HESS_GLOBAL), (256, _lib.HESS_GLOBAL))),
-    (192, ((192, _lib.HESS_LDS), (384, _lib.HESS_LDS), (128, _lib.HESS_GLOBAL), (192, _lib.HESS_GLOBAL), (256, _lib.HESS_GLOBAL))),
-    (264, ((640, _lib.HESS_LDS), (192, _lib.HESS_GLOBAL), (256, _lib.HESS_GLOBAL), (320, _lib.HESS_GLOBAL))),
+    (96, ((64, _lib.HESS_GLOBAL), (128, _lib.HESS_GLOBAL), (64, _lib.HESS_LDS), (128, _lib.HESS_LDS))),
+    (192, ((64, _lib.HESS_GLOBAL), (128, _lib.HESS_GLOBAL), (192, _lib.HESS_GLOBAL), (256, _lib.HESS_GLOBAL),
+           (192, _lib.HESS_LDS))),
+    (264, ((128, _lib.HESS_GLOBAL), (192, _lib.HESS_GLOBAL), (256, _lib.HESS_GLOBAL), (640, _lib.HESS_LDS))),
     (360, ((192, _lib.HESS_GLOBAL), (256, _lib.HESS_GLOBAL), (512, _lib.HESS_GLOBAL))),
     (1 << 20, ((256, _lib.HESS_GLOBAL), (512, _lib.HESS_GLOBAL), (640, _lib.HESS_GLOBAL))),
 )
 
 
-def autotune_buckets(db: DeviceBatch, x, classes=TUNE_CLASSES, sample: int = 512, reps: int = 1, seed: int = 0,
+def autotune_buckets(db: DeviceBatch, x, classes=TUNE_CLASSES, sample: int = 8192, reps: int = 1, seed: int = 0,
                      **opts):
     """Pick the block size (and hence waves per workgroup and workgroups per CU) and the
     Hessian placement of each size class by timing candidate launches on a seeded sample of
-    that class's conformers.
+    that class's conformers. A persistent launch holds up to 3,072 conformers at once on
+    MI355X (256 CUs x 12 waves), so a sample needs several times that many conformers to time
+    the steady state rather than the launch's tail; the default takes up to 8,192 per class.
 
     Returns a bucket tuple for NkRunner and the timings {bound: {(block, placement): ms}}.
     """

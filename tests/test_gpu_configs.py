@@ -1,9 +1,13 @@
 """GPU parity of the scaled configurations against the FP64 oracle, one test per kernel the
 default plans launch:
 
-* ragged batch (10-150 atoms): every default size bucket of the dense-BFGS plan -- the packed
-  LDS Hessian at 128 threads (n = 3N <= 96), the global Hessian at 256 threads (n <= 360) and
-  at 512 threads (n > 360) -- each compared with oracle BFGS from the same starts;
+* ragged batch (10-150 atoms): every default size bucket of the dense-BFGS plan -- the global
+  Hessian at 64 threads (n = 3N <= 96), 128 threads (n <= 192), 256 threads (n <= 360) and 512
+  threads (n > 360) -- and the packed LDS Hessian (128 threads, n <= 96), each compared with
+  oracle BFGS from the same starts;
+* per-size method choice: a plan with BFGS buckets up to n = 360 and an L-BFGS bucket above
+  gives each conformer exactly the result of the single-method plan it belongs to, and its
+  L-BFGS conformers agree with the oracle's L-BFGS;
 * flexible chains (118-365 atoms): L-BFGS with m = 10 and the history in global memory
   (n > 192), compared with the oracle's L-BFGS m = 10 iteration by iteration for the first
   iterations and at the end of a run with the reference pipeline's iteration budget;
@@ -72,6 +76,17 @@ def _oracle_floor(ob, x64, **kw):
     return float(agree.sum() / max(ok.sum(), 1)), e0, st0, x0
 
 
+RAGGED_CLASSES = ((0, 96), (96, 192), (192, 360), (360, 1 << 20))
+
+
+@pytest.fixture(scope="module")
+def ragged_oracle(ragged_sample):
+    """Oracle BFGS on the ragged sample, from its starts and from a 1e-6 A restart."""
+    wl = ragged_sample
+    ob = oracle.OracleBatch(wl.params, wl.conf_counts)
+    return ob, _oracle_restart(ob, wl.x.astype(np.float64), max_iters=2000, grad_tol=1e-4)
+
+
 @pytest.fixture(scope="module")
 def ragged_sample():
     """A seeded ragged sample with 20 molecules x 2 conformers in each default BFGS bucket."""
@@ -79,26 +94,26 @@ def ragged_sample():
     wl = workloads.build("ragged", n_mols=600, n_confs=2)
     n3 = 3 * np.array([p.n_atoms for p in wl.params])
     pick = []
-    for lo, hi in ((0, 96), (96, 360), (360, 1 << 20)):
+    for lo, hi in RAGGED_CLASSES:
         idx = np.nonzero((n3 > lo) & (n3 <= hi))[0]
         assert len(idx) >= 20, (lo, hi, len(idx))
         pick += list(idx[:20])
     return _select(wl, pick)
 
 
-def test_ragged_every_default_bucket_matches_oracle(ragged_sample):
+def test_ragged_every_default_bucket_matches_oracle(ragged_sample, ragged_oracle):
     from mlxmolkit_archive_amd import engine, _lib
     wl = ragged_sample
     db = engine.device_batch(wl.params, wl.conf_counts)
     xd = torch.from_numpy(wl.x).cuda()
-    runner = engine.NkRunner(db, max_iters=2000, grad_tol=1e-4, ls_grad=True)
-    got = sorted((l["opts"].block, l["opts"].hess_place) for l in runner.launches)
-    assert got == [(128, _lib.HESS_LDS), (256, _lib.HESS_GLOBAL), (512, _lib.HESS_GLOBAL)], got
+    # min_fill=0: every default bucket launches, however few conformers the sample gives it
+    runner = engine.NkRunner(db, max_iters=2000, grad_tol=1e-4, ls_grad=True, min_fill=0)
+    got = [(l["opts"].block, l["opts"].hess_place) for l in runner.launches]
+    G = _lib.HESS_GLOBAL
+    assert got == [(64, G), (128, G), (256, G), (512, G)], got
     res = runner.run(xd)
     torch.cuda.synchronize()
-    ob = oracle.OracleBatch(wl.params, wl.conf_counts)
-    x64 = wl.x.astype(np.float64)
-    e_ref, st_ref, agree, ok_ref, _ = _oracle_restart(ob, x64, max_iters=2000, grad_tol=1e-4)
+    ob, (e_ref, st_ref, agree, ok_ref, _) = ragged_oracle
     floor = float(agree.sum() / max(ok_ref.sum(), 1))
     e_gpu, _, _ = oracle.energy_grad_batch(ob, res.x.cpu().numpy().astype(np.float64))
     st = res.status.cpu().numpy()
@@ -115,7 +130,8 @@ def test_ragged_every_default_bucket_matches_oracle(ragged_sample):
     assert (np.abs(e_rep - e_gpu) <= 1e-3 + 2e-6 * sabs).all()
     d = e_gpu - e_ref
     print(f"\noracle self-agreement (1e-6 A restart): {floor:.3f}")
-    for name, lo, hi in (("LDS n<=96 x128", 0, 96), ("global n<=360 x256", 96, 360), ("global n>360 x512", 360, 1 << 20)):
+    for name, lo, hi in (("global n<=96 x64", 0, 96), ("global n<=192 x128", 96, 192),
+                         ("global n<=360 x256", 192, 360), ("global n>360 x512", 360, 1 << 20)):
         sel = (n3 > lo) & (n3 <= hi)
         ok = sel & (st == 0) & (st_ref == 0)
         within, lower, higher = _split(d, ok)
@@ -131,6 +147,110 @@ def test_ragged_every_default_bucket_matches_oracle(ragged_sample):
         assert within >= (min(0.95, fl) - 0.1) * ok.sum() and np.median(np.abs(d[ok])) < 2e-3
     ok = (st == 0) & (st_ref == 0)
     assert np.mean(np.abs(d[ok]) <= 0.1) >= min(0.97, floor - 0.03)
+
+
+def test_plan_query_and_bucket_folding(ragged_sample):
+    """mmffhip_minimize_nk_plan reports the persistent grid: at most one workgroup per
+    conformer, and for an unbounded batch the device's residency (twelve waves per CU for the
+    global-Hessian kernels up to 256 threads, one 512-thread workgroup per CU). With the default
+    min_fill the sample's buckets, each far from filling the device, fold into the last one."""
+    import ctypes as C
+    from mlxmolkit_archive_amd import engine, _lib
+    L = _lib.load()
+    wl = ragged_sample
+    n_cu = C.c_int32()
+    _lib.check(L.mmffhip_device_info(C.byref(n_cu), None, None))
+    G = _lib.HESS_GLOBAL
+    # the small molecules (n <= 96): a few KB of LDS per workgroup, so registers set residency
+    smallw = _select(wl, [m for m, p in enumerate(wl.params) if 3 * p.n_atoms <= 96])
+    dbs = engine.device_batch(smallw.params, smallw.conf_counts)
+    for block, per_cu in ((64, 12), (128, 6), (256, 3), (512, 1)):
+        o = engine.nk_options(max_iters=10, hess_place=G, block=block)
+        res = engine.resident_workgroups(dbs.desc, o)
+        assert res % n_cu.value == 0 and res >= per_cu * n_cu.value, (block, res)
+        g, b, lds, hp = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        _lib.check(L.mmffhip_minimize_nk_plan(C.byref(dbs.desc), C.byref(o), C.byref(g), C.byref(b), C.byref(lds),
+                                              C.byref(hp)))
+        assert g.value == min(dbs.n_confs, res) and b.value == block and hp.value == G
+        assert 0 < lds.value <= 160 * 1024
+    db = engine.device_batch(wl.params, wl.conf_counts)
+    r = engine.NkRunner(db, max_iters=10, grad_tol=1e-4)
+    assert [(l["opts"].block, l["n"]) for l in r.launches] == [(512, db.n_confs)]
+    r0 = engine.NkRunner(db, max_iters=10, grad_tol=1e-4, min_fill=0)
+    assert [l["opts"].block for l in r0.launches] == [64, 128, 256, 512]
+
+
+def test_ragged_lds_hessian_bucket_matches_oracle(ragged_sample, ragged_oracle):
+    """The packed LDS Hessian (an autotuner candidate for n <= 96) on the sample's small
+    conformers, against the same oracle run; its start geometries are the sample's."""
+    from mlxmolkit_archive_amd import engine, _lib
+    wl = ragged_sample
+    db = engine.device_batch(wl.params, wl.conf_counts)
+    xd = torch.from_numpy(wl.x).cuda()
+    n3 = 3 * wl.atoms_per_conf()
+    small = np.nonzero(n3 <= 96)[0].astype(np.int32)
+    runner = engine.NkRunner(db, order=small, buckets=((96, 128, _lib.HESS_LDS),), max_iters=2000, grad_tol=1e-4,
+                             ls_grad=True)
+    assert [(l["opts"].block, l["opts"].hess_place) for l in runner.launches] == [(128, _lib.HESS_LDS)]
+    res = runner.run(xd)
+    torch.cuda.synchronize()
+    ob, (e_ref, st_ref, agree, ok_ref, _) = ragged_oracle
+    e_gpu, _, _ = oracle.energy_grad_batch(ob, res.x.cpu().numpy().astype(np.float64))
+    st = res.status.cpu().numpy()
+    d = (e_gpu - e_ref)[small]
+    ok = (st[small] == 0) & (st_ref[small] == 0)
+    within, lower, higher = _split(d, ok)
+    print(f"\nLDS n<=96 x128: within 0.1 {within}/{ok.sum()}, GPU lower {lower}, higher {higher}, "
+          f"median |dE| {np.median(np.abs(d[ok])):.2e}")
+    assert ok.sum() >= 0.95 * len(small)
+    assert within >= 0.95 * ok.sum() and np.median(np.abs(d[ok])) < 2e-3
+
+
+def test_per_size_method_choice(ragged_sample):
+    """Buckets that name their method: BFGS up to n = 360, L-BFGS (m = 10, 128 threads, global
+    history) above. Each conformer's result is bit-identical to the plan that runs its own
+    method alone with the same block and placement (a conformer's workgroup does not depend on
+    the others), and the L-BFGS conformers agree with the oracle's L-BFGS m = 10 as often as
+    that oracle agrees with its own 1e-6 A restart, less a margin for the small class."""
+    from mlxmolkit_archive_amd import engine, _lib
+    wl = ragged_sample
+    db = engine.device_batch(wl.params, wl.conf_counts)
+    xd = torch.from_numpy(wl.x).cuda()
+    G = _lib.HESS_GLOBAL
+    kw = dict(max_iters=2000, grad_tol=1e-4, ls_grad=True, lbfgs_m=10)
+    mixed = engine.NkRunner(db, buckets=((360, 256, G, "bfgs"), (1 << 20, 128, G, "lbfgs")), **kw)
+    assert [(l["algo"], l["opts"].block) for l in mixed.launches] == [("bfgs", 256), ("lbfgs", 128)]
+    rm = mixed.run(xd)
+    torch.cuda.synchronize()
+    xm, em, sm = rm.x.cpu().numpy(), rm.energy.cpu().numpy().copy(), rm.status.cpu().numpy().copy()
+    n3 = 3 * wl.atoms_per_conf()
+    off = np.concatenate([[0], np.cumsum(n3)])
+    for algo, sel in (("bfgs", np.nonzero(n3 <= 360)[0]), ("lbfgs", np.nonzero(n3 > 360)[0])):
+        blk = 256 if algo == "bfgs" else 128
+        one = engine.NkRunner(db, order=sel.astype(np.int32), buckets=((1 << 20, blk, G),),
+                              use_lbfgs=algo == "lbfgs", **kw)
+        r1 = one.run(xd)
+        torch.cuda.synchronize()
+        x1 = r1.x.cpu().numpy()
+        for c in sel:
+            assert np.array_equal(x1[off[c]:off[c + 1]], xm[off[c]:off[c + 1]]), (algo, c)
+        assert np.array_equal(r1.energy.cpu().numpy()[sel], em[sel])
+        assert np.array_equal(r1.status.cpu().numpy()[sel], sm[sel])
+    big = np.nonzero(n3 > 360)[0]
+    sub = [wl.params[m] for m, cnt in enumerate(wl.conf_counts) for _ in range(cnt)]
+    ob = oracle.OracleBatch([sub[c] for c in big], [1] * len(big))
+    xb = np.concatenate([wl.x[off[c]:off[c + 1]] for c in big]).astype(np.float64)
+    e_ref, st_ref, agree, ok_ref, _ = _oracle_restart(ob, xb, max_iters=2000, grad_tol=1e-4, use_lbfgs=True,
+                                                      m_hist=10)
+    floor = float(agree.sum() / max(ok_ref.sum(), 1))
+    xg = np.concatenate([xm[off[c]:off[c + 1]] for c in big]).astype(np.float64)
+    e_gpu, _, _ = oracle.energy_grad_batch(ob, xg)
+    ok = (sm[big] == 0) & (st_ref == 0)
+    within, lower, higher = _split(e_gpu - e_ref, ok)
+    print(f"\nL-BFGS m=10 n>360 x128: within 0.1 {within}/{ok.sum()} (oracle vs its restart {floor:.3f}), "
+          f"GPU lower {lower}, higher {higher}")
+    assert ok.sum() >= 0.9 * len(big)
+    assert within >= (floor - 0.2) * ok.sum()
 
 
 @pytest.fixture(scope="module")
