@@ -305,8 +305,11 @@ def main():
     prof = ROOT / "profiles" / f"k_minimize_nk_{args.workload}_traffic.json"
     if default_shape and prof.exists():
         pj = json.loads(prof.read_text())
-        traffic = float(pj["dominant_launch_bytes"])
-        traffic_src = f"profiles/{prof.name} (rocprofv3 --pmc, separate run of this command)"
+        # only when the profiled launch is this run's dominant one (same block and conformers)
+        if (pj.get("dominant_launch_block") == launches[dom]["block"]
+                and pj.get("dominant_launch_conformers") == launches[dom]["conformers"]):
+            traffic = float(pj["dominant_launch_bytes"])
+            traffic_src = f"profiles/{prof.name} (rocprofv3 --pmc, separate run of this command)"
     conv_est = n_status0 * (frac if frac is not None else 1.0)
     value = conv_est / (kern_ms * 1e-3)
     ok = st == 0
