@@ -343,7 +343,7 @@ def main():
         "status0_fraction": round(n_status0 / wl.n_confs, 4),
         "within_0.1_fraction_sample": None if frac is None else round(frac, 4),
         "wall_ms_per_step": round(wall * 1e3, 3),
-        "host_s": {"generate_type_embed": round(t_gen, 1), "pack_upload_plan": round(t_pack, 2)},
+        "host_s": {"build_total": round(t_gen, 1), **wl.timings, "pack_upload_plan": round(t_pack, 2)},
         "launches": launches,
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK, 4),

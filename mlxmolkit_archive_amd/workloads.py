@@ -17,7 +17,8 @@ coordinate. Molecule and conformer counts can be scaled down for tests.
 from __future__ import annotations
 
 import sys
-from dataclasses import dataclass
+import time
+from dataclasses import dataclass, field
 from typing import List, Optional
 
 import numpy as np
@@ -41,6 +42,9 @@ class Workload:
     params: List[MMFFParams]
     conf_counts: List[int]
     x: np.ndarray  # float32 flat start coordinates (sum over conformers of 3N)
+    # wall seconds of the build's stages: molecule generation, typing (parameter tables),
+    # embedding (start geometries)
+    timings: dict = field(default_factory=dict)
 
     @property
     def n_confs(self) -> int:
@@ -84,7 +88,9 @@ def build(name: str, n_mols: Optional[int] = None, n_confs: Optional[int] = None
     fam, M, K = SPECS[name]
     M = M if n_mols is None else n_mols
     K = K if n_confs is None else n_confs
+    t0 = time.perf_counter()
     mols = molgen.generate(fam, M, seed=seed)
+    t1 = time.perf_counter()
     procs = _pool_size(M)
     if procs > 1:
         import multiprocessing as mp
@@ -96,6 +102,8 @@ def build(name: str, n_mols: Optional[int] = None, n_confs: Optional[int] = None
             params[i::procs] = part
     else:
         params = _typed((mols, variant))
+    t2 = time.perf_counter()
+    tm = {"generate_s": round(t1 - t0, 2), "type_s": round(t2 - t1, 2), "type_processes": procs}
     counts = [K] * M
     ids = np.arange(M * K, dtype=np.uint64)
     seeds = (np.uint64(seed) * np.uint64(1000003) + np.uint64(molgen._FAMILY_ID[fam]) * np.uint64(7919) * np.uint64(
@@ -104,7 +112,8 @@ def build(name: str, n_mols: Optional[int] = None, n_confs: Optional[int] = None
     if embedder == "device":
         from .embed import embed_device
         x = embed_device(params, z, counts, seeds=seeds, noise=noise).cpu().numpy()
-        return Workload(name=name, mols=mols, params=params, conf_counts=counts, x=x)
+        tm["embed_device_s"] = round(time.perf_counter() - t2, 2)
+        return Workload(name=name, mols=mols, params=params, conf_counts=counts, x=x, timings=tm)
     if embedder != "host":
         raise ValueError(f"embedder must be 'host' or 'device', not {embedder!r}")
     # large batches are embedded in slices of molecules, with a progress line per slice
@@ -117,4 +126,5 @@ def build(name: str, n_mols: Optional[int] = None, n_confs: Optional[int] = None
         if step < M:
             print(f"[workloads] {name}: embedded molecules {b}/{M}", file=sys.stderr, flush=True)
     x = np.concatenate(parts) if len(parts) > 1 else parts[0]
-    return Workload(name=name, mols=mols, params=params, conf_counts=counts, x=x)
+    tm["embed_host_s"] = round(time.perf_counter() - t2, 2)
+    return Workload(name=name, mols=mols, params=params, conf_counts=counts, x=x, timings=tm)
