@@ -93,6 +93,10 @@ def main():
     ap.add_argument("--oracle-restart", action="store_true",
                     help="also run the FP64 oracle from the sample's starts + N(0, 1e-6 A) and report how often it "
                          "agrees with itself within 0.1 kcal/mol (the floor any other path can reach)")
+    ap.add_argument("--oracle-fp32", action="store_true",
+                    help="also run the oracle with its FP32-emulation switch (coordinates, term energies, gradients "
+                         "and optimizer vectors rounded to float, FP64 sums) from the sample's starts and report how "
+                         "its minima compare with the FP64 oracle's and the GPU's")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--embedder", default="device", choices=["device", "host"],
                     help="start geometries: the restraint embedder on the GPU (default) or its host C form")
@@ -247,8 +251,11 @@ def main():
         n3s = n3[pick]
         offs = np.concatenate([[0], np.cumsum(n3s)])
 
-        def run_oracle(xa):
+        def run_oracle(xa, emulate=False):
             """The oracle on the sample, each conformer with the method the GPU ran it with."""
+            if emulate:
+                with oracle.fp32_emulation():
+                    return run_oracle(xa)
             e_o, st_o, cnt_o = np.zeros(S), np.zeros(S, np.int32), np.zeros((S, 3), np.int32)
             for dense in (True, False):
                 idx = np.nonzero(dense_conf[pick] == dense)[0]
@@ -293,6 +300,17 @@ def main():
             e_p, st_p, _ = run_oracle(xp)
             okp = (st_ref == 0) & (st_p == 0)
             sample_info["oracle_vs_restart_within_0.1"] = round(float(np.mean(np.abs(e_p - e_ref)[okp] <= 0.1)), 4)
+        if args.oracle_fp32:
+            e_f, st_f, cnt_f = run_oracle(xs, emulate=True)
+            okf = (st_ref == 0) & (st_f == 0)
+            df = (e_f - e_ref)[okf]
+            okg = ok0 & (st_f == 0)
+            sample_info["oracle_fp32emu"] = {
+                "status0": round(float(np.mean(st_f == 0)), 4),
+                "within_0.1_of_fp64_oracle": round(float(np.mean(np.abs(df) <= 0.1)), 4),
+                "lower": int((df < -0.1).sum()), "higher": int((df > 0.1).sum()),
+                "mean_iters_vs_fp64": [round(float(cnt_f[okf, 0].mean()), 1), round(float(cnt_ref[okf, 0].mean()), 1)],
+                "gpu_within_0.1_of_it": round(float(np.mean(np.abs(e_gpu_fp64 - e_f)[okg] <= 0.1)), 4)}
         log(f"cpu_baseline {t_cpu:.1f}s; within 0.1 kcal/mol: {within.sum()}/{ok0.sum()} status-0 sampled "
             f"(median |dE| {np.median(dE):.2e}; misses lower/higher {sample_info['miss_gpu_lower']}/"
             f"{sample_info['miss_gpu_higher']})")

@@ -496,6 +496,17 @@ static double mg_ele(const double *x, double *g, int i, int j, double qq, double
     return e;
 }
 
+/* FP32-storage emulation, an analysis switch that is off by default and used only to study
+ * how the HIP kernels' number formats move trajectories: when on, each TG-family term energy
+ * is rounded to float before the FP64 sum, the summed gradient is rounded to float, and the nk
+ * minimizers round coordinates to float at the start and after every step (the kernels keep
+ * coordinates and term results in FP32 and sum energies in FP64). Set it before a run; every
+ * thread of a batch reads the same setting. The dense inverse Hessian, its products and the
+ * L-BFGS vector updates are then rounded to float too (their dot products stay FP64). */
+static int g_fp32_emulation = 0;
+void oracle_set_fp32_emulation(int on) { g_fp32_emulation = on != 0; }
+static inline double rf(double v) { return g_fp32_emulation ? (double)(float)v : v; }
+
 double oracle_energy_grad(const oracle_mol *m, int family, int diel, const double *x, double *g,
                           uint32_t mask, double *et) {
     double e[7] = {0, 0, 0, 0, 0, 0, 0};
@@ -503,23 +514,28 @@ double oracle_energy_grad(const oracle_mol *m, int family, int diel, const doubl
     if (g) memset(g, 0, sizeof(double) * (size_t)n3);
     if (family == ORACLE_FAMILY_TG) {
         if (mask & T_BOND)
-            for (int t = 0; t < m->nb; t++) e[0] += tg_bond(x, g, m->b_i[t], m->b_j[t], m->b_kb[t], m->b_r0[t]);
+            for (int t = 0; t < m->nb; t++) e[0] += rf(tg_bond(x, g, m->b_i[t], m->b_j[t], m->b_kb[t], m->b_r0[t]));
         if (mask & T_ANGLE)
             for (int t = 0; t < m->na; t++)
-                e[1] += tg_angle(x, g, m->a_i[t], m->a_j[t], m->a_k[t], m->a_ka[t], m->a_t0[t], m->a_lin[t] != 0);
+                e[1] += rf(tg_angle(x, g, m->a_i[t], m->a_j[t], m->a_k[t], m->a_ka[t], m->a_t0[t], m->a_lin[t] != 0));
         if (mask & T_STRBEND)
             for (int t = 0; t < m->nsb; t++)
-                e[2] += tg_strbend(x, g, m->s_i[t], m->s_j[t], m->s_k[t], m->s_r0ij[t], m->s_r0kj[t], m->s_t0[t],
-                                   m->s_kij[t], m->s_kkj[t]);
+                e[2] += rf(tg_strbend(x, g, m->s_i[t], m->s_j[t], m->s_k[t], m->s_r0ij[t], m->s_r0kj[t], m->s_t0[t],
+                                      m->s_kij[t], m->s_kkj[t]));
         if (mask & T_OOP)
-            for (int t = 0; t < m->noop; t++) e[3] += tg_oop(x, g, m->o_i[t], m->o_j[t], m->o_k[t], m->o_l[t], m->o_koop[t]);
+            for (int t = 0; t < m->noop; t++)
+                e[3] += rf(tg_oop(x, g, m->o_i[t], m->o_j[t], m->o_k[t], m->o_l[t], m->o_koop[t]));
         if (mask & T_TORSION)
             for (int t = 0; t < m->nt; t++)
-                e[4] += tg_torsion(x, g, m->t_i[t], m->t_j[t], m->t_k[t], m->t_l[t], m->t_v1[t], m->t_v2[t], m->t_v3[t]);
+                e[4] += rf(tg_torsion(x, g, m->t_i[t], m->t_j[t], m->t_k[t], m->t_l[t], m->t_v1[t], m->t_v2[t],
+                                      m->t_v3[t]));
         if (mask & T_VDW)
-            for (int t = 0; t < m->nv; t++) e[5] += tg_vdw(x, g, m->v_i[t], m->v_j[t], m->v_r[t], m->v_eps[t]);
+            for (int t = 0; t < m->nv; t++) e[5] += rf(tg_vdw(x, g, m->v_i[t], m->v_j[t], m->v_r[t], m->v_eps[t]));
         if (mask & T_ELE)
-            for (int t = 0; t < m->ne; t++) e[6] += tg_ele(x, g, m->e_i[t], m->e_j[t], m->e_qq[t], diel, m->e_14[t] != 0);
+            for (int t = 0; t < m->ne; t++)
+                e[6] += rf(tg_ele(x, g, m->e_i[t], m->e_j[t], m->e_qq[t], diel, m->e_14[t] != 0));
+        if (g && g_fp32_emulation)
+            for (int i = 0; i < n3; i++) g[i] = rf(g[i]);
     } else {
         /* theta0 is stored in degrees; the mega packer converts to radians
          * (mmff_metal_kernel.py:283-286). */
@@ -635,7 +651,7 @@ static double line_search(const oracle_mol *m, int diel, const double *x_old, do
             memcpy(x, x_old, sizeof(double) * (size_t)n);
             return old_e;
         }
-        for (int i = 0; i < n; i++) x[i] = x_old[i] + lam * d[i];
+        for (int i = 0; i < n; i++) x[i] = rf(x_old[i] + lam * d[i]);
         double te = oracle_energy_grad(m, ORACLE_FAMILY_TG, diel, x, NULL, T_ALL, NULL);
         (*n_evals)++;
         if (te - old_e <= FUNCTOL * lam * slope) {
@@ -703,6 +719,7 @@ static int nk_minimize(const oracle_mol *mol, double *x, int max_iters, double g
     double *H = NULL;
     if (hist) h0 = NULL;
     int n_evals = 0, n_grads = 1, iters = 0;
+    for (int i = 0; i < n; i++) x[i] = rf(x[i]);
     /* initial energy + gradient, :115-135 */
     double energy = oracle_energy_grad(mol, ORACLE_FAMILY_TG, diel, x, g, T_ALL, NULL);
     double gs = GRAD_SCALE_INIT;
@@ -780,7 +797,7 @@ static int nk_minimize(const oracle_mol *mol, double *x, int max_iters, double g
                 double s = 0.0;
                 const double *row = H + (size_t)i * n;
                 for (int j = 0; j < n; j++) s += row[j] * dg[j];
-                hdg[i] = s;
+                hdg[i] = rf(s);
             }
             double fac = 0, fae = 0, sdg = 0, sxi = 0;
             for (int i = 0; i < n; i++) {
@@ -796,7 +813,7 @@ static int nk_minimize(const oracle_mol *mol, double *x, int max_iters, double g
                     double *row = H + (size_t)i * n;
                     for (int j = 0; j < n; j++) {
                         double aj = fi * xo[j] - fei * hdg[j];
-                        row[j] += fi * xo[i] * xo[j] - fei * hdg[i] * hdg[j] + fae * ai * aj;
+                        row[j] = rf(row[j] + (fi * xo[i] * xo[j] - fei * hdg[i] * hdg[j] + fae * ai * aj));
                     }
                 }
             }
@@ -804,7 +821,7 @@ static int nk_minimize(const oracle_mol *mol, double *x, int max_iters, double g
                 double s = 0.0;
                 const double *row = H + (size_t)i * n;
                 for (int j = 0; j < n; j++) s += row[j] * g[j];
-                d[i] = -s;
+                d[i] = -rf(s);
             }
         } else {
             /* L-BFGS history + two-loop, mmff_lbfgs_source_tg.metal:249-292 */
@@ -829,7 +846,8 @@ static int nk_minimize(const oracle_mol *mol, double *x, int max_iters, double g
                 for (int i = 0; i < n; i++) sq += S[i] * q[i];
                 double aj = hist->rho[slot] * sq;
                 hist->alpha[j] = aj;
-                for (int i = 0; i < n; i++) q[i] -= aj * Y[i];
+                const double ajv = rf(aj);
+                for (int i = 0; i < n; i++) q[i] = rf(q[i] - ajv * Y[i]);
             }
             if (hist->count > 0) {
                 int nw = (hist->idx - 1) % M;
@@ -840,8 +858,8 @@ static int nk_minimize(const oracle_mol *mol, double *x, int max_iters, double g
                     sy += S[i] * Y[i];
                     yy += Y[i] * Y[i];
                 }
-                double gam = sy / dmax(yy, 1e-30);
-                for (int i = 0; i < n; i++) q[i] *= gam;
+                double gam = rf(sy / dmax(yy, 1e-30));
+                for (int i = 0; i < n; i++) q[i] = rf(q[i] * gam);
             }
             for (int j = 0; j < hist->count; j++) {
                 int slot = (hist->idx - 1 - (hist->count - 1 - j)) % M;
@@ -851,7 +869,8 @@ static int nk_minimize(const oracle_mol *mol, double *x, int max_iters, double g
                 for (int i = 0; i < n; i++) yq += Y[i] * q[i];
                 double bj = hist->rho[slot] * yq;
                 double aj = hist->alpha[j];
-                for (int i = 0; i < n; i++) q[i] += (aj - bj) * S[i];
+                const double cf = rf(aj - bj);
+                for (int i = 0; i < n; i++) q[i] = rf(q[i] + cf * S[i]);
             }
             for (int i = 0; i < n; i++) d[i] = -q[i];
         }

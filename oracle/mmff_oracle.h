@@ -41,6 +41,10 @@ enum { T_BOND = 1, T_ANGLE = 2, T_STRBEND = 4, T_OOP = 8, T_TORSION = 16, T_VDW 
 /* Energy and (optionally) gradient of one conformer. x and g are (3N).
  * e_terms (optional, length 7) receives the per-term-type energies.
  * diel: 1 constant, 2 distance-dependent (TG family only; MEGA is always 1). */
+/* Analysis switch (default off): emulate the HIP kernels' FP32 coordinates and FP32 term
+ * results inside the FP64 oracle (see mmff_oracle.c). */
+void oracle_set_fp32_emulation(int on);
+
 double oracle_energy_grad(const oracle_mol *m, int family, int diel, const double *x,
                           double *g, uint32_t term_mask, double *e_terms);
 

@@ -78,6 +78,8 @@ def lib():
         L.oracle_multi_mol.restype = C.c_int
         L.oracle_multi_mol.argtypes = [vp, vp, vp, C.c_int, vp, C.c_int, C.c_int, C.c_double, C.c_int,
                                        C.c_double, C.c_double, vp, vp, vp, vp, C.c_int]
+        L.oracle_set_fp32_emulation.restype = None
+        L.oracle_set_fp32_emulation.argtypes = [C.c_int]
         L.oracle_multi_mol_ex.restype = C.c_int
         L.oracle_multi_mol_ex.argtypes = [vp, vp, vp, C.c_int, vp, C.c_int, C.c_int, C.c_double, C.c_int,
                                           C.c_double, C.c_double, vp, vp, vp, vp, C.c_int, C.c_int]
@@ -189,6 +191,20 @@ def energy_grad_batch(batch: OracleBatch, x, family=FAMILY_TG, diel=1, mask=T_AL
                                    family, diel, xx.ctypes.data, e.ctypes.data, g.ctypes.data, mask,
                                    et.ctypes.data, n_threads or default_threads())
     return e, g, et.reshape(-1, 7)
+
+
+class fp32_emulation:
+    """Context manager for the oracle's analysis switch: inside it, the TG energies and the nk
+    minimizers round coordinates, term energies and the gradient to float (the HIP kernels'
+    storage formats) while keeping FP64 sums. Used only to study trajectory differences."""
+
+    def __enter__(self):
+        lib().oracle_set_fp32_emulation(1)
+        return self
+
+    def __exit__(self, *exc):
+        lib().oracle_set_fp32_emulation(0)
+        return False
 
 
 def minimize_nk(batch: OracleBatch, x, max_iters=500, grad_tol=1e-4, use_lbfgs=False, m_hist=8, diel=1,
