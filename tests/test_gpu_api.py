@@ -151,6 +151,71 @@ def test_energy_grad_metal_shapes_and_mega_family():
     assert np.allclose(e, e_r, rtol=1e-5, atol=1e-4)
 
 
+def _pipeline_with_recorder(monkeypatch, smiles, k, **kw):
+    """Run generate_conformers_nk(run_mmff=True) and record every mmff_minimize_nk call the
+    MMFF stage makes (its inputs and outputs)."""
+    import mlxmolkit_archive_amd.minimize as mn
+    from mlxmolkit_archive_amd import generate_conformers_nk
+    calls = []
+    real = mn.mmff_minimize_nk
+
+    def rec(plist, counts, pos, **a):
+        x0 = pos.detach().cpu().numpy().copy() if hasattr(pos, "detach") else np.array(pos, copy=True)
+        out = real(plist, counts, pos, **a)
+        calls.append(dict(plist=plist, counts=list(counts), x0=x0, kw=a, out=out))
+        return out
+
+    monkeypatch.setattr(mn, "mmff_minimize_nk", rec)
+    return generate_conformers_nk(smiles, n_confs_per_mol=k, run_mmff=True, **kw), calls
+
+
+def test_pipeline_mmff_stage_matches_oracle(monkeypatch):
+    """The MMFF stage of the pipeline (conformer_pipeline_v2.py:292-326) against the oracle:
+    each chunk's minimizer call gets the reference's iteration budget 200 + 15 x atoms and
+    the L-BFGS switch (off below 150 atoms); the returned conformers are that call's outputs,
+    in order; and the FP64 oracle from the same starts reaches the same minima (0.1 kcal/mol)
+    for at least 90% of them. A batch with a 152-atom molecule switches every chunk to L-BFGS."""
+    smiles = ["CCO", "c1ccccc1C(=O)N", "CCCCCCCC", "CC(=O)OC1=CC=CC=C1C(=O)O", "CCN(CC)CC", "OCC(O)CO"]
+    res, calls = _pipeline_with_recorder(monkeypatch, smiles, 5, max_confs_per_batch=12)
+    assert res.total_conformers == 30 and len(calls) == res.n_batches >= 3
+    n_atoms = [mr.n_atoms for mr in res.molecules]
+    e_all, e_ref_all = [], []
+    got = {i: 0 for i in range(len(smiles))}
+    for call in calls:
+        kw = call["kw"]
+        assert kw["use_lbfgs"] is False
+        ob = oracle.OracleBatch(call["plist"], call["counts"])
+        assert kw["max_iters"] == 200 + 15 * max(p.n_atoms for p in call["plist"])
+        out_pos, e32, conv, info = call["out"]
+        _, e_ref, st_ref, _ = oracle.minimize_nk(ob, call["x0"].astype(np.float64), max_iters=kw["max_iters"],
+                                                 grad_tol=1e-4)
+        e_gpu, _, _ = oracle.energy_grad_batch(ob, out_pos.astype(np.float64))
+        ok = conv & (st_ref == 0)
+        e_all.append(e_gpu[ok])
+        e_ref_all.append(e_ref[ok])
+        # the pipeline hands back exactly this call's conformers, molecule by molecule
+        off = 0
+        c = 0
+        for p, cnt in zip(call["plist"], call["counts"]):
+            m = n_atoms.index(p.n_atoms) if n_atoms.count(p.n_atoms) == 1 else None
+            for _ in range(cnt):
+                if m is not None:
+                    mr = res.molecules[m]
+                    j = got[m]
+                    assert np.array_equal(mr.positions_3d[j].reshape(-1), out_pos[off:off + 3 * p.n_atoms])
+                    assert mr.energies[j] == float(info["energy_f64"][c]) and mr.converged[j] == bool(conv[c])
+                    got[m] += 1
+                off += 3 * p.n_atoms
+                c += 1
+    d = np.concatenate(e_all) - np.concatenate(e_ref_all)
+    print(f"\npipeline MMFF stage: {len(d)} conformers converged on both sides, within 0.1 "
+          f"{np.mean(np.abs(d) <= 0.1):.3f}, GPU lower {np.sum(d < -0.1)}, higher {np.sum(d > 0.1)}")
+    assert len(d) >= 27 and np.mean(np.abs(d) <= 0.1) >= 0.9
+    res2, calls2 = _pipeline_with_recorder(monkeypatch, ["C" * 50, "CCO"], 2)
+    assert calls2 and all(c["kw"]["use_lbfgs"] is True for c in calls2)
+    assert all(np.isfinite(e).all() for mr in res2.molecules for e in [np.array(mr.energies)])
+
+
 def test_pipeline_run_mmff():
     """tests/test_macrocycle_bounds.py:82-88: the pipeline runs with run_mmff=True."""
     from mlxmolkit_archive_amd import generate_conformers_nk
