@@ -10,7 +10,7 @@ from __future__ import annotations
 
 import ctypes as C
 from pathlib import Path
-from typing import List, Sequence
+from typing import List, Optional, Sequence
 
 import numpy as np
 
@@ -101,29 +101,37 @@ def embed(params_list: Sequence, z_list: Sequence[np.ndarray], conf_counts: Sequ
     return out
 
 
-def pack_restraints(params_list: Sequence, z_list: Sequence[np.ndarray]):
+def restraint_table(params, z: np.ndarray):
+    """One molecule's restraints in the device embedder's encoding: (ij uint32 = i | j << 16,
+    d float32 = d0 for an equality spring or -d_min for a lower-bound wall)."""
+    ri, rj, d, kind = restraints(params, np.asarray(z))
+    ij = (ri.astype(np.uint32) & 0xFFFF) | ((rj.astype(np.uint32) & 0xFFFF) << 16)
+    return ij.astype(np.uint32), np.where(kind == 0, d, -d).astype(np.float32)
+
+
+def pack_restraints(params_list: Sequence, z_list: Sequence[np.ndarray], tables: Optional[Sequence] = None):
     """Ragged restraint tables of a batch for the device embedder: (mol_desc int32 [M, 4] =
-    {n_atoms, offset, count, 0}, restr_ij uint32 = i | j << 16, restr_d float32 = d0 for an
-    equality spring or -d_min for a lower-bound wall)."""
+    {n_atoms, offset, count, 0}, restr_ij uint32, restr_d float32), from ``restraint_table``
+    per molecule (or from ``tables``, those per-molecule pairs made beforehand)."""
+    if tables is None:
+        tables = [restraint_table(p, z) for p, z in zip(params_list, z_list)]
     desc = np.zeros((len(params_list), 4), np.int32)
-    ij, dd = [], []
     off = 0
-    for k, (p, z) in enumerate(zip(params_list, z_list)):
-        ri, rj, d, kind = restraints(p, np.asarray(z))
-        desc[k] = (p.n_atoms, off, len(ri), 0)
-        ij.append((ri.astype(np.uint32) & 0xFFFF) | ((rj.astype(np.uint32) & 0xFFFF) << 16))
-        dd.append(np.where(kind == 0, d, -d).astype(np.float32))
-        off += len(ri)
-    cat = lambda parts, dt: np.concatenate(parts).astype(dt) if parts else np.zeros(0, dt)  # noqa: E731
-    return desc, cat(ij, np.uint32), cat(dd, np.float32)
+    for k, (p, (ij, dd)) in enumerate(zip(params_list, tables)):
+        desc[k] = (p.n_atoms, off, len(ij), 0)
+        off += len(ij)
+    cat = lambda parts, dt: np.concatenate(parts).astype(dt) if len(parts) else np.zeros(0, dt)  # noqa: E731
+    return desc, cat([t[0] for t in tables], np.uint32), cat([t[1] for t in tables], np.float32)
 
 
 def embed_device(params_list: Sequence, z_list: Sequence[np.ndarray], conf_counts: Sequence[int],
-                 seeds: Sequence[int], noise: float = 0.1, device: str = "cuda", stream=None):
+                 seeds: Sequence[int], noise: float = 0.1, device: str = "cuda", stream=None,
+                 tables: Optional[Sequence] = None):
     """The same seeded restraint embedding as ``embed``, run on the GPU (one workgroup per
     conformer, FP64, csrc/embed.hip). Returns the flat float32 coordinates as a device tensor,
     ready for the minimizer without a host round trip. Results agree with ``embed`` in
-    distribution, not bit for bit (device FP64 math and fixed-point gradient sums)."""
+    distribution, not bit for bit (device FP64 math and fixed-point gradient sums).
+    ``tables``: optional per-molecule ``restraint_table`` results made beforehand."""
     from . import _lib
     torch = _lib.require_gpu()
     L = _lib.load()
@@ -139,7 +147,7 @@ def embed_device(params_list: Sequence, z_list: Sequence[np.ndarray], conf_count
         raise ValueError("batch exceeds 2^31 coordinates")
     sd = np.ascontiguousarray(np.asarray(seeds, dtype=np.uint64))
     assert len(sd) == len(conf_mol)
-    desc, ij, dd = pack_restraints(params_list, z_list)
+    desc, ij, dd = pack_restraints(params_list, z_list, tables)
     out = torch.empty(total, dtype=torch.float32, device=device)
     if len(conf_mol) == 0:
         return out

@@ -302,27 +302,30 @@ def _chain_backbone(b: _Builder, rng, n_heavy: int, macrocycles: int) -> None:
                 break
 
 
+def generate_one(family: str, m: int, seed: int = SEED_BASE) -> Mol:
+    """Molecule ``m`` of a workload family: a function of (seed, family, m) alone, so any
+    subset of a family's molecules can be generated independently (in worker processes)."""
+    rng = np.random.default_rng([seed, _FAMILY_ID[family], m])
+    if family == "small":
+        nh = int(rng.integers(10, 26))
+        return gen_molecule(rng, nh, max_rings=1, name=f"small-{m}")
+    if family == "medium":
+        nh = int(rng.integers(15, 41))
+        return gen_molecule(rng, nh, max_rings=1, name=f"medium-{m}")
+    if family == "ragged":
+        nh = int(round(math.exp(rng.uniform(math.log(8), math.log(60)))))
+        return gen_molecule(rng, nh, max_rings=3, allow_fused=True, name=f"ragged-{m}")
+    if family == "chain":
+        nh = int(rng.integers(60, 151))
+        return gen_molecule(rng, nh, max_rings=1, macrocycles=2, chain_linkers=True, name=f"chain-{m}")
+    raise ValueError(family)
+
+
 def generate(family: str, n_mols: int, seed: int = SEED_BASE) -> List[Mol]:
     """Deterministic list of molecules of one workload family."""
-    out = []
-    for m in range(n_mols):
-        rng = np.random.default_rng([seed, _FAMILY_ID[family], m])
-        if family == "small":
-            nh = int(rng.integers(10, 26))
-            mol = gen_molecule(rng, nh, max_rings=1, name=f"small-{m}")
-        elif family == "medium":
-            nh = int(rng.integers(15, 41))
-            mol = gen_molecule(rng, nh, max_rings=1, name=f"medium-{m}")
-        elif family == "ragged":
-            nh = int(round(math.exp(rng.uniform(math.log(8), math.log(60)))))
-            mol = gen_molecule(rng, nh, max_rings=3, allow_fused=True, name=f"ragged-{m}")
-        elif family == "chain":
-            nh = int(rng.integers(60, 151))
-            mol = gen_molecule(rng, nh, max_rings=1, macrocycles=2, chain_linkers=True, name=f"chain-{m}")
-        else:
-            raise ValueError(family)
-        out.append(mol)
-    return out
+    if family not in _FAMILY_ID:
+        raise ValueError(family)
+    return [generate_one(family, m, seed) for m in range(n_mols)]
 
 
 _FAMILY_ID = {"small": 1, "medium": 2, "ragged": 3, "chain": 4}

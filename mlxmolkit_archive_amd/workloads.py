@@ -65,6 +65,19 @@ def _typed(args):
     return out
 
 
+def _gen_typed(args):
+    """Worker: generate the given molecules of a family, type them and make their embedder
+    restraint tables (each is a function of its own index, so workers need no shared state)."""
+    fam, idx, seed, variant, with_tables = args
+    mols = [molgen.generate_one(fam, int(m), seed) for m in idx]
+    params = _typed((mols, variant))
+    tables = None
+    if with_tables:
+        from .embed import restraint_table
+        tables = [restraint_table(p, m.atomic_numbers()) for p, m in zip(params, mols)]
+    return mols, params, tables
+
+
 def _pool_size(n_mols: int) -> int:
     """Worker processes for typing: only for large batches. The workers are started fresh
     (spawned, not forked), so a GPU runtime already initialised in this process, by torch or
@@ -89,21 +102,29 @@ def build(name: str, n_mols: Optional[int] = None, n_confs: Optional[int] = None
     M = M if n_mols is None else n_mols
     K = K if n_confs is None else n_confs
     t0 = time.perf_counter()
-    mols = molgen.generate(fam, M, seed=seed)
-    t1 = time.perf_counter()
     procs = _pool_size(M)
+    tables = None
     if procs > 1:
+        # generation, typing and the device embedder's restraint tables in spawned workers
         import multiprocessing as mp
-        chunks = [mols[i::procs] for i in range(procs)]
         with mp.get_context("spawn").Pool(procs) as pool:
-            parts = pool.map(_typed, [(c, variant) for c in chunks])
-        params = [None] * M
-        for i, part in enumerate(parts):
-            params[i::procs] = part
+            parts = pool.map(_gen_typed, [(fam, np.arange(i, M, procs), seed, variant, embedder == "device")
+                                          for i in range(procs)])
+        mols, params = [None] * M, [None] * M
+        tables = [None] * M if embedder == "device" else None
+        for i, (pm, pp, pt) in enumerate(parts):
+            mols[i::procs] = pm
+            params[i::procs] = pp
+            if tables is not None:
+                tables[i::procs] = pt
+        t1 = t2 = time.perf_counter()
+        tm = {"generate_type_tables_s": round(t2 - t0, 2), "processes": procs}
     else:
+        mols = molgen.generate(fam, M, seed=seed)
+        t1 = time.perf_counter()
         params = _typed((mols, variant))
-    t2 = time.perf_counter()
-    tm = {"generate_s": round(t1 - t0, 2), "type_s": round(t2 - t1, 2), "type_processes": procs}
+        t2 = time.perf_counter()
+        tm = {"generate_s": round(t1 - t0, 2), "type_s": round(t2 - t1, 2)}
     counts = [K] * M
     ids = np.arange(M * K, dtype=np.uint64)
     seeds = (np.uint64(seed) * np.uint64(1000003) + np.uint64(molgen._FAMILY_ID[fam]) * np.uint64(7919) * np.uint64(
@@ -111,7 +132,7 @@ def build(name: str, n_mols: Optional[int] = None, n_confs: Optional[int] = None
     z = [m.atomic_numbers() for m in mols]
     if embedder == "device":
         from .embed import embed_device
-        x = embed_device(params, z, counts, seeds=seeds, noise=noise).cpu().numpy()
+        x = embed_device(params, z, counts, seeds=seeds, noise=noise, tables=tables).cpu().numpy()
         tm["embed_device_s"] = round(time.perf_counter() - t2, 2)
         return Workload(name=name, mols=mols, params=params, conf_counts=counts, x=x, timings=tm)
     if embedder != "host":
