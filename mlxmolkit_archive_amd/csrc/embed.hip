@@ -15,6 +15,9 @@
 //   * N(0, noise) is added to every coordinate (Box-Muller on the same stream).
 // Arithmetic is FP64. Gradients are summed in 64-bit fixed point (2^-32) in LDS and energies
 // are reduced in a fixed order, so a conformer's result does not depend on thread timing.
+// Coordinates and gradient sums sit in LDS as four planes (one per dimension, stride `ps`
+// elements), so the lanes of a wave, which take consecutive restraints and hence mostly
+// consecutive second atoms, read consecutive 8-byte words.
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 #include <stdint.h>
@@ -66,10 +69,10 @@ struct EmbedMol {
 };
 
 // Restraint energy (this lane's part) and gradient (into g, zeroed by the caller) at x
-// (stride 4 per atom; the first `dim` coordinates are used).
+// (coordinate c of atom a at [c * ps + a]; the first `dim` coordinates are used).
 template <int NT>
 __device__ double restraint_eg(const uint32_t* rij, const float* rd, const EmbedMol& m, const double* x,
-                               unsigned long long* g, int dim, double w4) {
+                               unsigned long long* g, int ps, int dim, double w4) {
   double e = 0.0;
   for (int t = threadIdx.x; t < m.n_restr; t += NT) {
     const uint32_t ij = rij[t];
@@ -79,7 +82,7 @@ __device__ double restraint_eg(const uint32_t* rij, const float* rd, const Embed
     const double d0 = fabs((double)dd);
     double dv[4] = {0.0, 0.0, 0.0, 0.0}, d2 = 0.0;
     for (int c = 0; c < dim; c++) {
-      dv[c] = x[4 * i + c] - x[4 * j + c];
+      dv[c] = x[c * ps + i] - x[c * ps + j];
       d2 += dv[c] * dv[c];
     }
     if (wall && d2 + 1e-12 >= d0 * d0) continue;  // an inactive wall costs no square root
@@ -90,15 +93,15 @@ __device__ double restraint_eg(const uint32_t* rij, const float* rd, const Embed
     const double f = 2.0 * k * diff / d;
     for (int c = 0; c < dim; c++) {
       const unsigned long long q = efx(f * dv[c]);
-      atomicAdd(&g[4 * i + c], q);
-      atomicSub(&g[4 * j + c], q);
+      atomicAdd(&g[c * ps + i], q);
+      atomicSub(&g[c * ps + j], q);
     }
   }
   if (dim == 4 && w4 > 0.0) {
     for (int a = threadIdx.x; a < m.n_atoms; a += NT) {
-      const double w = x[4 * a + 3];
+      const double w = x[3 * ps + a];
       e += w4 * w * w;
-      atomicAdd(&g[4 * a + 3], efx(2.0 * w4 * w));
+      atomicAdd(&g[3 * ps + a], efx(2.0 * w4 * w));
     }
   }
   return e;
@@ -108,28 +111,28 @@ __device__ double restraint_eg(const uint32_t* rij, const float* rd, const Embed
 // accepted step (every lane takes the same branch: the sums are identical on all lanes).
 template <int NT>
 __device__ void relax(const uint32_t* rij, const float* rd, const EmbedMol& m, double*& x, unsigned long long*& g,
-                      double*& xt, unsigned long long*& gt, double* red, int dim, double w4, int steps) {
-  const int n4 = 4 * m.n_atoms;
+                      double*& xt, unsigned long long*& gt, double* red, int ps, int dim, double w4, int steps) {
+  const int n4 = 4 * ps;
   double step = 0.002;
   for (int i = threadIdx.x; i < n4; i += NT) g[i] = 0ull;
   __syncthreads();
-  double e = block_sum<NT>(restraint_eg<NT>(rij, rd, m, x, g, dim, w4), red);
+  double e = block_sum<NT>(restraint_eg<NT>(rij, rd, m, x, g, ps, dim, w4), red);
   for (int s = 0; s < steps; s++) {
     for (int a = threadIdx.x; a < m.n_atoms; a += NT) {
       double gv[4], gn = 0.0;
       for (int c = 0; c < dim; c++) {
-        gv[c] = efx_val(g[4 * a + c]);
+        gv[c] = efx_val(g[c * ps + a]);
         gn += gv[c] * gv[c];
       }
       gn = sqrt(gn);
       double sc = step;
       if (gn * sc > 0.3) sc = 0.3 / gn;
-      for (int c = 0; c < dim; c++) xt[4 * a + c] = x[4 * a + c] - sc * gv[c];
-      for (int c = dim; c < 4; c++) xt[4 * a + c] = x[4 * a + c];
+      for (int c = 0; c < dim; c++) xt[c * ps + a] = x[c * ps + a] - sc * gv[c];
+      for (int c = dim; c < 4; c++) xt[c * ps + a] = x[c * ps + a];
     }
     for (int i = threadIdx.x; i < n4; i += NT) gt[i] = 0ull;
     __syncthreads();
-    const double et = block_sum<NT>(restraint_eg<NT>(rij, rd, m, xt, gt, dim, w4), red);
+    const double et = block_sum<NT>(restraint_eg<NT>(rij, rd, m, xt, gt, ps, dim, w4), red);
     if (et <= e) {
       double* tx = x;
       x = xt;
@@ -154,7 +157,8 @@ __global__ void __launch_bounds__(NT) k_embed(const int32_t* desc, const uint32_
   extern __shared__ __align__(16) unsigned char smem[];
   const int c = blockIdx.x;
   if (c >= n_confs) return;
-  const int mx4 = 4 * max_atoms;
+  const int ps = max_atoms;  // plane stride
+  const int mx4 = 4 * ps;
   double* red = (double*)smem;
   double* xa = red + 16;
   double* xb = xa + mx4;
@@ -167,22 +171,23 @@ __global__ void __launch_bounds__(NT) k_embed(const int32_t* desc, const uint32_
   const uint64_t seed = seeds[c];
   const int n = m.n_atoms;
   const double box = 1.5 * cbrt((double)n) + 1.0;
-  for (int i = threadIdx.x; i < 4 * n; i += NT) xa[i] = (unif(seed, (uint64_t)i) * 2.0 - 1.0) * box;
+  // draw k = 4a + c is coordinate c of atom a (the host embedder's order)
+  for (int i = threadIdx.x; i < 4 * n; i += NT) xa[(i & 3) * ps + (i >> 2)] = (unif(seed, (uint64_t)i) * 2.0 - 1.0) * box;
   __syncthreads();
   double *x = xa, *xt = xb;
   unsigned long long *g = ga, *gt = gb;
-  relax<NT>(r_ij, r_d, m, x, g, xt, gt, red, 4, 0.0, 250);
-  relax<NT>(r_ij, r_d, m, x, g, xt, gt, red, 4, 2.0, 150);
-  for (int a = threadIdx.x; a < n; a += NT) x[4 * a + 3] = 0.0;  // drop the fourth coordinate
+  relax<NT>(r_ij, r_d, m, x, g, xt, gt, red, ps, 4, 0.0, 250);
+  relax<NT>(r_ij, r_d, m, x, g, xt, gt, red, ps, 4, 2.0, 150);
+  for (int a = threadIdx.x; a < n; a += NT) x[3 * ps + a] = 0.0;  // drop the fourth coordinate
   __syncthreads();
-  relax<NT>(r_ij, r_d, m, x, g, xt, gt, red, 3, 0.0, 200);
+  relax<NT>(r_ij, r_d, m, x, g, xt, gt, red, ps, 3, 0.0, 200);
   float* o = x_out + conf_off[c];
   const uint64_t base = 4 * (uint64_t)n;
   for (int q = threadIdx.x; q < 3 * n; q += NT) {
     double u1 = unif(seed, base + 2 * (uint64_t)q), u2 = unif(seed, base + 2 * (uint64_t)q + 1);
     if (u1 < 1e-300) u1 = 1e-300;
     const double z = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
-    o[q] = (float)(x[4 * (q / 3) + q % 3] + (double)noise * z);
+    o[q] = (float)(x[(q % 3) * ps + q / 3] + (double)noise * z);
   }
 }
 
