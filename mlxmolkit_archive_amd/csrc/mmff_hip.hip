@@ -691,6 +691,11 @@ constexpr double GRAD_CAP = 10.0;
 constexpr int LBFGS_MAX_M = 16;
 // pending rank-2 updates the global-Hessian BFGS pass applies on the fly before writing H back
 constexpr int HPEND_MAX = 4;
+// the same for the 512-thread kernel, which has the registers for more (one workgroup per CU,
+// 233 VGPRs without spills at six): H is then written every sixth iteration. Ragged workload,
+// n > 360 class: 1692.5 ms against 1726.2 ms with four and 1716.2 ms with eight
+constexpr int HPEND_LARGE = 6;
+__host__ __device__ constexpr int hpend_for(int nt) { return nt >= 512 ? HPEND_LARGE : HPEND_MAX; }
 
 struct NkParams {
   const float* x_in;
@@ -733,7 +738,7 @@ __host__ __device__ inline size_t nk_lds_bytes(int algo, int hm, int nt, int max
   b += (size_t)max_n * 4 * 4;   // xs, d, xo, hd
   b += (size_t)max_n * 16;      // V = {dg, g, pending xi, pending hd}
   b += (size_t)max_n * 8;       // per-lane coefficients of the pending update
-  if (algo == MMFFHIP_BFGS && hm == 1) b += (size_t)(HPEND_MAX - 1) * max_n * 16;  // more pending updates
+  if (algo == MMFFHIP_BFGS && hm == 1) b += (size_t)(hpend_for(nt) - 1) * max_n * 16;  // more pending updates
   if (algo == MMFFHIP_LBFGS) {
     if (hlds) b += 2 * (size_t)m * max_n * 4;
   } else if (hlds) {
@@ -776,10 +781,10 @@ __global__ void __launch_bounds__(NT, (HM == 1 && NT <= 256) ? 3 : 1) k_minimize
   float* xo = d + mn;   // previous x, then xi = x - x_old
   float* hd = xo + mn;  // H dg (BFGS) / two-loop q (L-BFGS)
   float2* AB = (float2*)(hd + mn);  // pending update U(t, j) = AB[t].x xi_j + AB[t].y hd_j
-  // global-Hessian BFGS keeps up to HPEND_MAX pending updates (slot 0 in SH/AB, slot k >= 1
-  // in XS) and writes H back once every HPEND_MAX iterations
+  // global-Hessian BFGS keeps up to NPM pending updates (slot 0 in SH/AB, slot k >= 1 in XS)
+  // and writes H back once every NPM iterations
   constexpr bool LAZY = HM == 1 && ALGO == MMFFHIP_BFGS;
-  constexpr int NPM = LAZY ? HPEND_MAX : 1;
+  constexpr int NPM = LAZY ? hpend_for(NT) : 1;
   float2* XS = AB + mn;
   auto sh_slot = [&](int k) { return k == 0 ? SH : XS + (size_t)(k - 1) * 2 * mn; };
   auto ab_slot = [&](int k) { return k == 0 ? AB : XS + (size_t)(k - 1) * 2 * mn + mn; };
