@@ -1230,42 +1230,77 @@ __global__ void __launch_bounds__(NT, (HM == 1 && NT <= 256) ? 3 : 1) k_minimize
           if (hist_count < M) hist_count++;
         }
         for (int i = tid; i < n; i += NT) hd[i] = VG[i].y;
-        __syncthreads();
-        for (int j = hist_count - 1; j >= 0; j--) {
-          int slot = (hist_idx - 1 - (hist_count - 1 - j)) % M;
-          if (slot < 0) slot += M;
-          double sq = 0.0;
-          for (int i = tid; i < n; i += NT) sq += (double)S[(size_t)slot * mn + i] * hd[i];
-          sq = R.sum(sq);
-          const double aj = rho[slot] * sq;
-          if (tid == 0) alf[j] = aj;
-          const float af = (float)aj;
-          for (int i = tid; i < n; i += NT) hd[i] -= af * Y[(size_t)slot * mn + i];
-        }
-        if (hist_count > 0) {
-          int nw = (hist_idx - 1) % M;
-          if (nw < 0) nw += M;
-          double v2b[2] = {0.0, 0.0};
+        __syncthreads();  // rho visible
+        // The two-loop recursion with each axpy fused into the next dot product's pass (and
+        // gamma's dots into the first pass, the gamma scaling into the last axpy of the first
+        // loop): the same FP32 updates of q and the same FP64 dot products, element by
+        // element and lane by lane, so results are bit-identical to running the passes one
+        // by one, in 2m + 1 passes over the history instead of 4m + 3.
+        const int h = hist_count;
+        if (h == 0) {
+          for (int i = tid; i < n; i += NT) d[i] = -hd[i];
+        } else {
+          int sl = (hist_idx - 1) % M;  // newest entry (j = h - 1)
+          if (sl < 0) sl += M;
+          double v3[3] = {0.0, 0.0, 0.0};  // s_{h-1}.q, and s.y, y.y of the newest pair for gamma
           for (int i = tid; i < n; i += NT) {
-            const double s_ = S[(size_t)nw * mn + i], y_ = Y[(size_t)nw * mn + i];
-            v2b[0] += s_ * y_;
-            v2b[1] += y_ * y_;
+            const double s_ = S[(size_t)sl * mn + i], y_ = Y[(size_t)sl * mn + i];
+            v3[0] += s_ * hd[i];
+            v3[1] += s_ * y_;
+            v3[2] += y_ * y_;
           }
-          R.template run<2, 0>(v2b);
-          const float gam = (float)(v2b[0] / fmax(v2b[1], 1e-30));
-          for (int i = tid; i < n; i += NT) hd[i] *= gam;
-        }
-        __syncthreads();  // alf / rho visible
-        for (int j = 0; j < hist_count; j++) {
-          int slot = (hist_idx - 1 - (hist_count - 1 - j)) % M;
-          if (slot < 0) slot += M;
+          R.template run<3, 0>(v3);
+          const float gam = (float)(v3[1] / fmax(v3[2], 1e-30));
+          double aj = rho[sl] * v3[0];
+          if (tid == 0) alf[h - 1] = aj;
+          for (int j = h - 1; j >= 1; j--) {  // q -= alpha_j y_j, then s_{j-1}.q
+            const int sn = sl == 0 ? M - 1 : sl - 1;
+            const float af = (float)aj;
+            double sq = 0.0;
+            for (int i = tid; i < n; i += NT) {
+              const float q = hd[i] - af * Y[(size_t)sl * mn + i];
+              hd[i] = q;
+              sq += (double)S[(size_t)sn * mn + i] * q;
+            }
+            sq = R.sum(sq);
+            aj = rho[sn] * sq;
+            if (tid == 0) alf[j - 1] = aj;
+            sl = sn;
+          }
+          // q -= alpha_0 y_0, r = gamma q, then y_0.r (sl is entry 0's slot)
           double yq = 0.0;
-          for (int i = tid; i < n; i += NT) yq += (double)Y[(size_t)slot * mn + i] * hd[i];
-          yq = R.sum(yq);
-          const float cf = (float)(alf[j] - rho[slot] * yq);
-          for (int i = tid; i < n; i += NT) hd[i] += cf * S[(size_t)slot * mn + i];
+          {
+            const float af = (float)aj;
+            for (int i = tid; i < n; i += NT) {
+              const float y_ = Y[(size_t)sl * mn + i];
+              float q = hd[i] - af * y_;
+              q *= gam;
+              hd[i] = q;
+              yq += (double)y_ * q;
+            }
+          }
+          yq = R.sum(yq);  // its barrier also makes alf visible
+          for (int j = 0; j < h; j++) {  // r += (alpha_j - beta_j) s_j, then y_{j+1}.r
+            const float cf = (float)(alf[j] - rho[sl] * yq);
+            const int sn = sl + 1 == M ? 0 : sl + 1;
+            if (j + 1 < h) {
+              double y2 = 0.0;
+              for (int i = tid; i < n; i += NT) {
+                const float r = hd[i] + cf * S[(size_t)sl * mn + i];
+                hd[i] = r;
+                y2 += (double)Y[(size_t)sn * mn + i] * r;
+              }
+              yq = R.sum(y2);
+            } else {
+              for (int i = tid; i < n; i += NT) {
+                const float r = hd[i] + cf * S[(size_t)sl * mn + i];
+                hd[i] = r;
+                d[i] = -r;
+              }
+            }
+            sl = sn;
+          }
         }
-        for (int i = tid; i < n; i += NT) d[i] = -hd[i];
         __syncthreads();
       }
     }
