@@ -1484,40 +1484,66 @@ __global__ void __launch_bounds__(NT) k_mm_chunk(KBatch B, MMState S, int steps,
   double E = S.e_new[c];
   __syncthreads();
   for (int st = 0; st < steps; st++) {
-    // two-loop direction, mmff_metal_optimizer.py:47-74 (history oldest -> newest)
+    // two-loop direction, mmff_metal_optimizer.py:47-74 (history oldest -> newest), with each
+    // axpy fused into the next dot product's pass, gamma's dots into the first pass and the
+    // gamma scaling into the first loop's last axpy: the same FP32 updates of q and FP64 dot
+    // products as running the passes one by one (bit-identical), in 2 len + 1 passes.
     for (int i = tid; i < n; i += NT) q[i] = gs[i];
-    for (int h = len - 1; h >= 0; h--) {
-      const int sl = (head + h) % M;
-      double sq = 0.0;
-      for (int i = tid; i < n; i += NT) sq += (double)S.S[sl * T + off + i] * q[i];
-      sq = R.sum(sq);
-      const double a = S.rho[(size_t)sl * B.n_confs + c] * sq;
-      if (tid == 0) alf[h] = a;
-      const float af = (float)a;
-      for (int i = tid; i < n; i += NT) q[i] -= af * S.Y[sl * T + off + i];
-    }
-    double gam = 1.0;
     if (len > 0) {
-      const int sl = (head + len - 1) % M;
-      double v2[2] = {0.0, 0.0};
+      int sl = (head + len - 1) % M;  // newest entry
+      double v3[3] = {0.0, 0.0, 0.0};
       for (int i = tid; i < n; i += NT) {
         const double s = S.S[sl * T + off + i], y = S.Y[sl * T + off + i];
-        v2[0] += s * y;
-        v2[1] += y * y;
+        v3[0] += s * q[i];
+        v3[1] += s * y;
+        v3[2] += y * y;
       }
-      R.template run<2, 0>(v2);
-      gam = v2[0] / (v2[1] + 1e-30);
-    }
-    const float gf = (float)gam;
-    for (int i = tid; i < n; i += NT) q[i] *= gf;
-    __syncthreads();
-    for (int h = 0; h < len; h++) {
-      const int sl = (head + h) % M;
-      double yr = 0.0;
-      for (int i = tid; i < n; i += NT) yr += (double)S.Y[sl * T + off + i] * q[i];
-      yr = R.sum(yr);
-      const float cf = (float)(alf[h] - S.rho[(size_t)sl * B.n_confs + c] * yr);
-      for (int i = tid; i < n; i += NT) q[i] += cf * S.S[sl * T + off + i];
+      R.template run<3, 0>(v3);
+      const float gf = (float)(v3[1] / (v3[2] + 1e-30));
+      double a = S.rho[(size_t)sl * B.n_confs + c] * v3[0];
+      if (tid == 0) alf[len - 1] = a;
+      for (int h = len - 1; h >= 1; h--) {  // q -= alpha_h y_h, then s_{h-1}.q
+        const int sn = (head + h - 1) % M;
+        const float af = (float)a;
+        double sq = 0.0;
+        for (int i = tid; i < n; i += NT) {
+          const float qv = q[i] - af * S.Y[sl * T + off + i];
+          q[i] = qv;
+          sq += (double)S.S[sn * T + off + i] * qv;
+        }
+        sq = R.sum(sq);
+        a = S.rho[(size_t)sn * B.n_confs + c] * sq;
+        if (tid == 0) alf[h - 1] = a;
+        sl = sn;
+      }
+      double yr = 0.0;  // q -= alpha_0 y_0, q *= gamma, then y_0.q
+      {
+        const float af = (float)a;
+        for (int i = tid; i < n; i += NT) {
+          const float y = S.Y[sl * T + off + i];
+          float qv = q[i] - af * y;
+          qv *= gf;
+          q[i] = qv;
+          yr += (double)y * qv;
+        }
+      }
+      yr = R.sum(yr);  // its barrier also makes alf visible
+      for (int h = 0; h < len; h++) {  // q += (alpha_h - beta_h) s_h, then y_{h+1}.q
+        const float cf = (float)(alf[h] - S.rho[(size_t)sl * B.n_confs + c] * yr);
+        const int sn = (head + h + 1) % M;
+        if (h + 1 < len) {
+          double y2 = 0.0;
+          for (int i = tid; i < n; i += NT) {
+            const float r = q[i] + cf * S.S[sl * T + off + i];
+            q[i] = r;
+            y2 += (double)S.Y[sn * T + off + i] * r;
+          }
+          yr = R.sum(y2);
+        } else {
+          for (int i = tid; i < n; i += NT) q[i] += cf * S.S[sl * T + off + i];
+        }
+        sl = sn;
+      }
     }
     __syncthreads();
     // d = -q with the per-atom cap of max_step (:365-373); x_new = x + d
