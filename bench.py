@@ -102,9 +102,10 @@ def main():
                     help="start geometries: the restraint embedder on the GPU (default) or its host C form")
     ap.add_argument("--graph", action="store_true",
                     help="replay the step from a HIP graph (then only whole-step times are measured)")
-    ap.add_argument("--engine", default="nk", choices=["nk", "multimol"],
+    ap.add_argument("--engine", default="nk", choices=["nk", "multimol", "fused"],
                     help="nk: in-kernel BFGS/L-BFGS of the run_mmff path (headline); multimol: the lock-step "
-                         "L-BFGS of mmff_optimize_metal_multi_mol run on the device")
+                         "L-BFGS of mmff_optimize_metal_multi_mol run on the device; fused: the in-kernel "
+                         "Barzilai-Borwein descent of mmff_optimize_metal_fused_multi_mol (always max_iters)")
     ap.add_argument("--precond", action="store_true",
                     help="start BFGS from the learned per-atom inverse-Hessian blocks (trained first on a "
                          "separately seeded 300 x 2 training set)")
@@ -126,6 +127,8 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.engine == "multimol":
         return bench_multimol(args, wl, t_gen)
+    if args.engine == "fused":
+        return bench_fused(args, wl, t_gen)
     log(f"workload {args.workload}: {len(wl.params)} molecules, {wl.n_confs} conformers, "
         f"atoms {wl.atoms_per_conf().min()}-{wl.atoms_per_conf().max()} (built in {t_gen:.1f}s)")
     if args.max_iters <= 0:  # conformer_pipeline_v2.py:123-126,169-170
@@ -446,7 +449,8 @@ def bench_multimol(args, wl, t_gen):
                "cores": int(threads), "kind": "port",
                "sample": f"{S} seeded conformers of the same workload, FP64 oracle lock-step L-BFGS, {t_cpu:.1f}s wall"}
         info = {"sample": S, "converged_in_sample": int(ok.sum()), "within_0.1_in_sample": int((ok & (dE <= 0.1)).sum()),
-                "median_dE": float(np.median(dE))}
+                "median_dE": float(np.median(dE)),
+                "within_0.1_of_fp64_bfgs_minimum": _vs_bfgs_minimum(oracle, ob, xs, xgs, threads)}
     value = conv.sum() * (frac if frac is not None else 1.0) / (kern_ms * 1e-3)
     out_line = {
         "metric": "converged conformers/s at |dE|<=0.1 kcal/mol vs FP64; mean iters to converge",
@@ -464,6 +468,87 @@ def bench_multimol(args, wl, t_gen):
         "roofline": {"bound": "hbm", "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK, 4), "traffic": None,
                      "note": "per iteration T + 8n (E+G) + 16mn + 48n (two-loop, m = 10) bytes per conformer"},
+        "cpu_baseline": cpu, "check": info,
+    }
+    print(json.dumps(out_line), flush=True)
+
+
+def _vs_bfgs_minimum(oracle, ob, xs, xg, threads):
+    """Fraction of the sample whose GPU end point is within 0.1 kcal/mol (MMFF94 TG energy,
+    FP64) of the FP64 oracle's converged dense-BFGS minimum from the same start (maxIters 2000):
+    how close an engine gets to a converged minimization, as the reference's own benchmark
+    compares each method with a full host minimization."""
+    _, e_min, st_min, _ = oracle.minimize_nk(ob, xs, max_iters=2000, grad_tol=1e-4, n_threads=threads)
+    e_g, _, _ = oracle.energy_grad_batch(ob, xg, family=oracle.FAMILY_TG, n_threads=threads)
+    return round(float(np.mean(np.abs(e_g - e_min) <= 0.1)), 4)
+
+
+def bench_fused(args, wl, t_gen):
+    """The fused engine: in-kernel Barzilai-Borwein steepest descent of
+    mmff_optimize_metal_fused_multi_mol (max_step 0.3, grad_lr 0.1, MEGA formulas), which runs
+    every conformer for exactly max_iters iterations and has no convergence test. A conformer
+    counts as converged when its end point is within 0.1 kcal/mol of the FP64 oracle's
+    converged BFGS minimum from the same start (measured on the sample)."""
+    import torch
+    from mlxmolkit_archive_amd import engine
+    db = engine.device_batch(wl.params, wl.conf_counts)
+    x_dev = torch.from_numpy(wl.x).cuda()
+    for _ in range(args.warmup):
+        engine.bb_fused(db, x_dev, max_iters=args.max_iters, max_step=0.3, grad_lr=0.1)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ms = []
+    for _ in range(args.steps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        xo, eo = engine.bb_fused(db, x_dev, max_iters=args.max_iters, max_step=0.3, grad_lr=0.1)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    kern_ms = float(np.mean(ms))
+    xg = xo.cpu().numpy()
+    frac, cpu, info = None, None, {}
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle
+        rng = np.random.default_rng(20261202)
+        S = min(args.sample, wl.n_confs)
+        pick = np.sort(rng.choice(wl.n_confs, size=S, replace=False))
+        conf_mol = np.repeat(np.arange(len(wl.params)), wl.conf_counts)
+        n3i = 3 * wl.atoms_per_conf()
+        off = np.concatenate([[0], np.cumsum(n3i)[:-1]])
+        ob = oracle.OracleBatch([wl.params[conf_mol[c]] for c in pick], [1] * S)
+        xs = np.concatenate([wl.x[off[c]:off[c] + n3i[c]] for c in pick]).astype(np.float64)
+        xgs = np.concatenate([xg[off[c]:off[c] + n3i[c]] for c in pick]).astype(np.float64)
+        threads = args.cpu_threads or oracle.default_threads()
+        log(f"cpu_baseline: FP64 oracle BB descent on {S} sampled conformers, {threads} threads ...")
+        t0 = time.perf_counter()
+        xr, e_ref = oracle.bb_fused(ob, xs, max_iters=args.max_iters, max_step=0.3, grad_lr=0.1, n_threads=threads)
+        t_cpu = time.perf_counter() - t0
+        e_fp64, _, _ = oracle.energy_grad_batch(ob, xgs, family=oracle.FAMILY_MEGA, n_threads=threads)
+        frac = _vs_bfgs_minimum(oracle, ob, xs, xgs, threads)
+        e_ref_tg, _, _ = oracle.energy_grad_batch(ob, xr, family=oracle.FAMILY_TG, n_threads=threads)
+        _, e_min, _, _ = oracle.minimize_nk(ob, xs, max_iters=2000, grad_tol=1e-4, n_threads=threads)
+        cpu_conv = float(np.sum(np.abs(e_ref_tg - e_min) <= 0.1))
+        cpu = {"value": round(cpu_conv / t_cpu, 3), "unit": "converged conformers/s", "cores": int(threads),
+               "kind": "port", "sample": f"{S} seeded conformers of the same workload, FP64 oracle BB descent "
+                                         f"({args.max_iters} iterations), {t_cpu:.1f}s wall"}
+        info = {"sample": S, "gpu_vs_fp64_same_algorithm_within_0.1": round(float(np.mean(np.abs(e_fp64 - e_ref) <= 0.1)), 4),
+                "median_dE_vs_fp64_same_algorithm": float(np.median(np.abs(e_fp64 - e_ref))),
+                "within_0.1_of_fp64_bfgs_minimum": frac}
+    value = wl.n_confs * (frac if frac is not None else 1.0) / (kern_ms * 1e-3)
+    out_line = {
+        "metric": "converged conformers/s at |dE|<=0.1 kcal/mol vs FP64; mean iters to converge",
+        "value": round(value, 2), "unit": "conformers/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(kern_ms, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 terms, f64 energy sums",
+        "data": "synthetic: seeded molecules, typed by the built-in MMFF94-style typer, embedded + N(0,0.1A) noise",
+        "config": {"workload": f"{args.workload}: {len(wl.params)} molecules x {wl.conf_counts[0]} conformers",
+                   "algorithm": "fused in-kernel Barzilai-Borwein descent (MEGA terms), max_step 0.3, grad_lr 0.1",
+                   "max_iters": args.max_iters,
+                   "parallelism": "1 GPU, one workgroup per conformer, one launch"},
+        "mean_iters_to_converge": None, "iters_used": args.max_iters,
+        "within_0.1_fraction_sample": frac,
         "cpu_baseline": cpu, "check": info,
     }
     print(json.dumps(out_line), flush=True)
