@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define MMFFHIP_ABI_VERSION 1
+#define MMFFHIP_ABI_VERSION 2
 
 enum mmffhip_status {
   MMFFHIP_OK = 0,
@@ -107,6 +107,11 @@ typedef struct {
                               when that keeps the workgroups per CU. Ignored when they do not
                               fit. A conformer whose staged tables need more gets status 2 and
                               E = NaN */
+  const int32_t *xcd_offsets; /* optional host array of 9 rising offsets from 0 to n_confs that
+                                 split `order` into 8 per-XCD work queues: a workgroup on XCD x
+                                 (workgroups go to the 8 XCDs round-robin) takes conformers from
+                                 order[xcd_offsets[x] .. xcd_offsets[x+1]) first, then from the
+                                 other queues. NULL = one queue over `order` */
 } mmffhip_nk_opts;
 
 int mmffhip_abi_version(void);

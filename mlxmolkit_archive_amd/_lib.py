@@ -10,7 +10,7 @@ from typing import Optional
 
 from ._build import lib_path
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 ERR = {0: "ok", -1: "invalid argument", -2: "HIP runtime error", -3: "size limit", -4: "no device"}
 
@@ -37,7 +37,7 @@ class NkOpts(C.Structure):
     _fields_ = [
         ("algo", C.c_int32), ("max_iters", C.c_int32), ("grad_tol", C.c_float), ("diel", C.c_int32),
         ("lbfgs_m", C.c_int32), ("hess_place", C.c_int32), ("block", C.c_int32), ("ls_grad", C.c_int32),
-        ("precond", _vp), ("table_lds_bytes", C.c_int32),
+        ("precond", _vp), ("table_lds_bytes", C.c_int32), ("xcd_offsets", _vp),
     ]
 
 

@@ -704,7 +704,7 @@ struct NkParams {
   int32_t* status;
   int32_t* counters;
   const int32_t* order;
-  int32_t* queue;     // one int, zeroed before launch
+  int32_t* queue;     // work counters, zeroed before launch: one, or one per XCD
   float* hglob;       // per-workgroup global slots (H or L-BFGS history)
   size_t slot_floats;
   const float* precond;
@@ -715,6 +715,8 @@ struct NkParams {
   int max_n;          // 3 * max_atoms rounded up to a multiple of 4
   int ls_grad;
   int tbytes;         // LDS bytes reserved for staged term tables (TL kernels)
+  int xcd_on;         // order is split into 8 per-XCD queues at xcd_off
+  int32_t xcd_off[9];
 };
 
 // Packed upper-triangle storage of a symmetric n x n matrix with every column padded to a
@@ -802,7 +804,28 @@ __global__ void __launch_bounds__(NT, (HM == 1 && NT <= 256) ? 3 : 1) k_minimize
   Reducer<NT> R{red, 0};
 
   for (;;) {
-    if (tid == 0) s_int[0] = atomicAdd(P.queue, 1);
+    if (tid == 0) {
+      int kk;
+      if (P.xcd_on) {
+        // the queue of this workgroup's XCD first (workgroups are dispatched to the 8 XCDs
+        // round-robin), then the others' until every queue is drained
+        const int x0 = blockIdx.x & 7;
+        kk = B.n_confs;
+        for (int t = 0; t < 8; t++) {
+          const int x = (x0 + t) & 7;
+          const int cnt = P.xcd_off[x + 1] - P.xcd_off[x];
+          if (cnt <= 0) continue;
+          const int v = atomicAdd(P.queue + x, 1);
+          if (v < cnt) {
+            kk = P.xcd_off[x] + v;
+            break;
+          }
+        }
+      } else {
+        kk = atomicAdd(P.queue, 1);
+      }
+      s_int[0] = kk;
+    }
     __syncthreads();
     const int k = s_int[0];
     __syncthreads();
@@ -1851,7 +1874,7 @@ int run_nk(const mmffhip_batch* b, const mmffhip_nk_opts* o, const NkPlan& pl, c
            double* e_out, int32_t* status_out, int32_t* counters, const int32_t* order, void* workspace,
            hipStream_t s) {
   int32_t* queue = (int32_t*)workspace;
-  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, queue, 1, 0);
+  hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(64), 0, s, queue, 8, 0);
   HIP_TRY(hipGetLastError());
   NkParams P;
   P.x_in = x_in;
@@ -1871,6 +1894,20 @@ int run_nk(const mmffhip_batch* b, const mmffhip_nk_opts* o, const NkPlan& pl, c
   P.max_n = pl.max_n;
   P.ls_grad = o->ls_grad;
   P.tbytes = pl.tbytes;
+  P.xcd_on = 0;
+  for (int x = 0; x < 9; x++) P.xcd_off[x] = 0;
+  if (o->xcd_offsets) {
+    if (!order) return fail(MMFFHIP_ERR_ARG, "xcd_offsets need an order");
+    int prev = 0;
+    for (int x = 0; x < 9; x++) {
+      const int v = o->xcd_offsets[x];
+      if (v < prev || v > b->n_confs || (x == 0 && v != 0) || (x == 8 && v != b->n_confs))
+        return fail(MMFFHIP_ERR_ARG, "xcd_offsets must rise from 0 to n_confs");
+      P.xcd_off[x] = v;
+      prev = v;
+    }
+    P.xcd_on = 1;
+  }
   KBatch kb = to_kbatch(b);
   launch_nk(o->algo, kb, P, pl, s);
   HIP_TRY(hipGetLastError());

@@ -30,7 +30,7 @@ def test_abi_version_and_struct_sizes():
     L = _lib.load()
     assert L.mmffhip_abi_version() == _lib.ABI_VERSION
     assert ctypes.sizeof(_lib.Batch) == 16 * 8 + 3 * 4 + 4  # 16 pointers + 3 int32 (+ padding)
-    assert ctypes.sizeof(_lib.NkOpts) == 8 * 4 + 8 + 8
+    assert ctypes.sizeof(_lib.NkOpts) == 8 * 4 + 8 + 8 + 8  # ... precond, table_lds_bytes (+ pad), xcd_offsets
 
 
 def test_oracle_is_not_linked_by_the_product():

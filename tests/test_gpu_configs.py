@@ -180,6 +180,27 @@ def test_plan_query_and_bucket_folding(ragged_sample):
     assert [l["opts"].block for l in r0.launches] == [64, 128, 256, 512]
 
 
+def test_xcd_queues_give_the_same_results(ragged_sample):
+    """Per-XCD work queues change only which workgroup takes which conformer: positions,
+    energies, statuses and counters equal those of one queue bit for bit, for every default
+    bucket and for L-BFGS."""
+    from mlxmolkit_archive_amd import engine
+    wl = ragged_sample
+    db = engine.device_batch(wl.params, wl.conf_counts)
+    xd = torch.from_numpy(wl.x).cuda()
+    for kw in (dict(), dict(use_lbfgs=True, lbfgs_m=10)):
+        outs = []
+        for xq in (False, True):
+            r = engine.NkRunner(db, max_iters=2000, grad_tol=1e-4, ls_grad=True, min_fill=0, xcd_queues=xq, **kw)
+            if xq:
+                assert all(l["xcd_off"] is not None and l["xcd_off"][-1] == l["n"] for l in r.launches)
+            res = r.run(xd)
+            torch.cuda.synchronize()
+            outs.append([t.cpu().numpy().copy() for t in (res.x, res.energy, res.status, res.counters)])
+        for a, b in zip(*outs):
+            assert np.array_equal(a, b)
+
+
 def test_ragged_lds_hessian_bucket_matches_oracle(ragged_sample, ragged_oracle):
     """The packed LDS Hessian (an autotuner candidate for n <= 96) on the sample's small
     conformers, against the same oracle run; its start geometries are the sample's."""
