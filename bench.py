@@ -357,7 +357,7 @@ def main():
                                 + (" from learned H0" if pc is not None else ""), "max_iters": args.max_iters,
                    "grad_tol": args.grad_tol, "atoms_min_max": [int(wl.atoms_per_conf().min()),
                                                                 int(wl.atoms_per_conf().max())],
-                   "parallelism": "1 GPU, one workgroup per conformer, persistent size-bucketed launches",
+                   "parallelism": "1 GPU, one workgroup per conformer, persistent size-bucketed launches, per-XCD work queues",
                    "launch": "eager, one launch per size bucket, in sequence" if graph is None
                              else "HIP graph replay"},
         "mean_iters_to_converge": round(mean_iters, 1),
