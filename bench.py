@@ -88,6 +88,9 @@ def main():
     ap.add_argument("--lbfgs-above", type=int, default=0,
                     help="choose the method per conformer: dense BFGS up to n = 3N of this value, L-BFGS "
                          "(m = --lbfgs-m, 128 threads, history in global memory) above; 0 = off")
+    ap.add_argument("--hess-place", default="default", choices=["default", "global", "sym"],
+                    help="where the dense-BFGS buckets keep the inverse Hessian: default = engine.DEFAULT_BUCKETS; "
+                         "global = the full matrix in HBM; sym = one 4x4 block per pair of row quads in HBM")
     ap.add_argument("--sample", type=int, default=2048, help="conformers checked by the FP64 oracle")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--oracle-restart", action="store_true",
@@ -153,10 +156,16 @@ def main():
     db = engine.device_batch(wl.params, wl.conf_counts)
     x_dev = torch.from_numpy(wl.x).cuda()
     buckets = None
+    base = engine.DEFAULT_BUCKETS
+    if args.hess_place != "default":
+        hp = {"global": _lib.HESS_GLOBAL, "sym": _lib.HESS_GLOBAL_SYM}[args.hess_place]
+        base = tuple((b, blk, hp) for b, blk, _ in base)
+        if not args.lbfgs:
+            buckets = base
     if args.lbfgs_above > 0:
         assert not args.lbfgs and pc is None, "--lbfgs-above mixes methods; it excludes --lbfgs and --precond"
         buckets = []
-        for b, blk, pl in engine.DEFAULT_BUCKETS:
+        for b, blk, pl in base:
             buckets.append((min(b, args.lbfgs_above), blk, pl, "bfgs"))
             if b >= args.lbfgs_above:
                 break
@@ -216,7 +225,7 @@ def main():
     for j, ln in enumerate(runner.launches):
         sel = ln["order"].cpu().numpy()
         launches.append({"conformers": int(ln["n"]), "max_n": int(ln["max_n"]), "block": int(ln["opts"].block),
-                         "hessian": {1: "lds", 2: "global"}.get(int(ln["opts"].hess_place), "auto"),
+                         "hessian": {1: "lds", 2: "global", 3: "global-packed", 4: "global-sym"}[ln["placement"]],
                          "ms": None if launch_ms is None else round(launch_ms[j], 3),
                          "bytes": float(Bc[sel].sum()), "flops": float(Fc[sel].sum())})
     if launch_ms is not None:
@@ -328,7 +337,8 @@ def main():
         pj = json.loads(prof.read_text())
         # only when the profiled launch is this run's dominant one (same block and conformers)
         if (pj.get("dominant_launch_block") == launches[dom]["block"]
-                and pj.get("dominant_launch_conformers") == launches[dom]["conformers"]):
+                and pj.get("dominant_launch_conformers") == launches[dom]["conformers"]
+                and pj.get("dominant_launch_hessian", "global") == launches[dom]["hessian"]):
             traffic = float(pj["dominant_launch_bytes"])
             traffic_src = f"profiles/{prof.name} (rocprofv3 --pmc, separate run of this command)"
     conv_est = n_status0 * (frac if frac is not None else 1.0)

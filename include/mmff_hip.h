@@ -41,12 +41,15 @@ enum mmffhip_family { MMFFHIP_FAMILY_TG = 0, MMFFHIP_FAMILY_MEGA = 1 };
 enum mmffhip_algo { MMFFHIP_BFGS = 0, MMFFHIP_LBFGS = 1 };
 /* where k_minimize_nk keeps the dense inverse Hessian (L-BFGS: its history): AUTO = LDS when it
  * fits, else HBM; LDS = packed upper triangle in LDS; GLOBAL = full matrix in HBM; GLOBAL_PACKED
- * = packed upper triangle in HBM (dense BFGS; L-BFGS treats it as GLOBAL) */
+ * = packed upper triangle in HBM (dense BFGS; L-BFGS treats it as GLOBAL); GLOBAL_SYM = one
+ * 4x4 block per pair of row quads in HBM, half the bytes of GLOBAL (dense BFGS without a
+ * learned start; otherwise treated as GLOBAL) */
 enum mmffhip_hess_place {
   MMFFHIP_HESS_AUTO = 0,
   MMFFHIP_HESS_LDS = 1,
   MMFFHIP_HESS_GLOBAL = 2,
-  MMFFHIP_HESS_GLOBAL_PACKED = 3
+  MMFFHIP_HESS_GLOBAL_PACKED = 3,
+  MMFFHIP_HESS_GLOBAL_SYM = 4
 };
 
 /* Device-side ragged batch. Per-molecule term tables are shared by that molecule's
@@ -130,7 +133,8 @@ int mmffhip_minimize_nk_workspace(const mmffhip_batch *b, const mmffhip_nk_opts 
 
 /* The launch mmffhip_minimize_nk would make for this batch and options: the persistent grid
  * (resident workgroups, at most one per conformer), threads per workgroup, LDS bytes per
- * workgroup and the Hessian placement chosen (MMFFHIP_HESS_LDS / _GLOBAL / _GLOBAL_PACKED).
+ * workgroup and the Hessian placement chosen (MMFFHIP_HESS_LDS / _GLOBAL / _GLOBAL_PACKED /
+ * _GLOBAL_SYM).
  * block, lds_bytes and hess_place may be NULL. No counterpart in the reference, which
  * launches one threadgroup per conformer (mlxmolkit/mmff_minimize.py:310-311,340-341): callers
  * use it to size their size buckets against the device's residency. */

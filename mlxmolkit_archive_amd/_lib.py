@@ -18,7 +18,7 @@ FAMILY_TG = 0
 FAMILY_MEGA = 1
 BFGS = 0
 LBFGS = 1
-HESS_AUTO, HESS_LDS, HESS_GLOBAL, HESS_GLOBAL_PACKED = 0, 1, 2, 3
+HESS_AUTO, HESS_LDS, HESS_GLOBAL, HESS_GLOBAL_PACKED, HESS_GLOBAL_SYM = 0, 1, 2, 3, 4
 MM_BATCH = 1  # mmffhip_multimol_run_ex flag: the mmff_optimize_metal_batch form
 
 _vp = C.c_void_p

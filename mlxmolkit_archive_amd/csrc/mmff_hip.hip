@@ -729,9 +729,25 @@ __host__ __device__ inline int pcol(int c) {
 }
 __host__ __device__ inline size_t packed_floats(int n) { return (size_t)pcol(n); }
 
+// Symmetric blocked storage in HBM: the matrix is cut into 4x4 blocks over the nq = n/4 row
+// quads, and only one block per unordered quad pair {L, M} is kept. Block (s, L) holds rows
+// 4L..4L+3 and columns 4M..4M+3 with M = (L + s) mod nq, for the cyclic distances
+// s = 0 .. nq/2 (for even nq the pairs at distance nq/2 are kept once, by L < nq/2); its row r
+// is float4 number (4 s + r) nq + L, so the lanes that walk consecutive L of one distance read
+// consecutive 16-byte words. sym_slot_floats(max_n) floats hold the largest matrix.
+__host__ __device__ inline int sym_dists(int nq) { return (nq >> 1) + 1; }
+__host__ __device__ inline size_t sym_slot_floats(int max_n) {
+  const int nq = max_n >> 2;
+  return (size_t)16 * sym_dists(nq) * nq;
+}
+// LDS floats of one wave's private H v accumulators: up to 64/nq copies of the 4 nq entries
+// when nq <= 64 (at most 256 floats), one copy otherwise
+__host__ __device__ inline int sym_acc_floats(int max_n) { return max_n > 256 ? max_n : 256; }
+
 // LDS footprint (bytes) of the nk kernel
-// hm: Hessian placement (0 packed LDS, 1 full HBM, 2 packed HBM); only the full-HBM BFGS
-// kernel keeps the extra pending-update slots
+// hm: Hessian placement (0 packed LDS, 1 full HBM, 2 packed HBM, 3 symmetric blocked HBM);
+// only the full and symmetric HBM BFGS kernels keep the extra pending-update slots, and the
+// symmetric one adds its per-wave accumulators
 __host__ __device__ inline size_t nk_lds_bytes(int algo, int hm, int nt, int max_n, int m, size_t tbytes = 0) {
   const bool hlds = hm == 0;
   size_t nw = (size_t)nt / 64;
@@ -740,7 +756,8 @@ __host__ __device__ inline size_t nk_lds_bytes(int algo, int hm, int nt, int max
   b += (size_t)max_n * 4 * 4;   // xs, d, xo, hd
   b += (size_t)max_n * 16;      // V = {dg, g, pending xi, pending hd}
   b += (size_t)max_n * 8;       // per-lane coefficients of the pending update
-  if (algo == MMFFHIP_BFGS && hm == 1) b += (size_t)(hpend_for(nt) - 1) * max_n * 16;  // more pending updates
+  if (algo == MMFFHIP_BFGS && (hm == 1 || hm == 3)) b += (size_t)(hpend_for(nt) - 1) * max_n * 16;  // more pending updates
+  if (algo == MMFFHIP_BFGS && hm == 3) b += nw * (size_t)sym_acc_floats(max_n) * 4;
   if (algo == MMFFHIP_LBFGS) {
     if (hlds) b += 2 * (size_t)m * max_n * 4;
   } else if (hlds) {
@@ -765,9 +782,10 @@ __device__ __forceinline__ float h_init(const float* pc, float psc, int i, int j
 // use is small, and the extra resident conformers hide memory latency better than the
 // registers the compiler would otherwise use (medium workload: 172.6 against 178.5 ms).
 template <int ALGO, int HM, int TL, int NT>
-__global__ void __launch_bounds__(NT, (HM == 1 && NT <= 256) ? 3 : 1) k_minimize_nk(KBatch B, NkParams P) {
+__global__ void __launch_bounds__(NT, ((HM == 1 || HM == 3) && NT <= 256) ? 3 : 1) k_minimize_nk(KBatch B, NkParams P) {
   constexpr bool HLDS = HM == 0;    // LDS-resident (also selects the LDS L-BFGS history)
-  constexpr bool PACKED = HM != 1;  // packed upper triangle, select read + column-pair write-back
+  constexpr bool PACKED = HM == 0 || HM == 2;  // packed upper triangle, select read + column-pair write-back
+  constexpr bool SYM = HM == 3 && ALGO == MMFFHIP_BFGS;  // symmetric 4x4 blocks in HBM
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int NW = NT / 64;
   const int tid = threadIdx.x;
@@ -785,7 +803,7 @@ __global__ void __launch_bounds__(NT, (HM == 1 && NT <= 256) ? 3 : 1) k_minimize
   float2* AB = (float2*)(hd + mn);  // pending update U(t, j) = AB[t].x xi_j + AB[t].y hd_j
   // global-Hessian BFGS keeps up to NPM pending updates (slot 0 in SH/AB, slot k >= 1 in XS)
   // and writes H back once every NPM iterations
-  constexpr bool LAZY = HM == 1 && ALGO == MMFFHIP_BFGS;
+  constexpr bool LAZY = (HM == 1 || HM == 3) && ALGO == MMFFHIP_BFGS;
   constexpr int NPM = LAZY ? hpend_for(NT) : 1;
   float2* XS = AB + mn;
   auto sh_slot = [&](int k) { return k == 0 ? SH : XS + (size_t)(k - 1) * 2 * mn; };
@@ -794,6 +812,12 @@ __global__ void __launch_bounds__(NT, (HM == 1 && NT <= 256) ? 3 : 1) k_minimize
   float* H = HLDS ? big : P.hglob + (size_t)blockIdx.x * P.slot_floats;
   size_t big_floats = 0;
   if (HLDS) big_floats = ALGO == MMFFHIP_BFGS ? packed_floats(mn) : 2 * (size_t)P.m * mn;
+  // symmetric storage: per-wave H v accumulators (kept zero between passes)
+  const int sacc_w = sym_acc_floats(mn);
+  if (SYM) {
+    big_floats = (size_t)NW * sacc_w;
+    for (int i = tid; i < NW * sacc_w; i += NT) big[i] = 0.f;
+  }
   unsigned char* tstage = (unsigned char*)(big + big_floats);
   const int M = P.m;
   float* S = H;
@@ -860,6 +884,15 @@ __global__ void __launch_bounds__(NT, (HM == 1 && NT <= 256) ? 3 : 1) k_minimize
 
     for (int i = tid; i < n; i += NT) xs[i] = P.x_in[off + i];
     for (int i = tid; i < n; i += NT) gacc[i] = 0ull;
+    if (SYM) {
+      // the padding rows of the last quad: zero gradient and update vectors, so that the
+      // padding elements of H stay zero and add nothing to H v
+      for (int i = n + tid; i < ((n + 3) & ~3); i += NT) {
+        VG[i] = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int k = 0; k < NPM; k++) sh_slot(k)[i] = ab_slot(k)[i] = make_float2(0.f, 0.f);
+      }
+    }
     float psc = 1.0f;
     auto init_h = [&]() {
       if (ALGO != MMFFHIP_BFGS) return;
@@ -867,6 +900,21 @@ __global__ void __launch_bounds__(NT, (HM == 1 && NT <= 256) ? 3 : 1) k_minimize
         for (int col = tid; col < n; col += NT) {
           float* cp = H + pcol(col);
           for (int i = 0; i < ((col + 4) & ~3); i++) cp[i] = i <= col ? h_init(pc, psc, i, col) : 0.0f;
+        }
+      } else if (SYM) {
+        const int nq = (n + 3) >> 2;
+        float4* H4 = (float4*)H;
+        for (int f = tid; f < 4 * sym_dists(nq) * nq; f += NT) {
+          const int sr = f / nq, L = f - sr * nq, r = sr & 3, s = sr >> 2;
+          const int M = L + s < nq ? L + s : L + s - nq;
+          const int jr = 4 * L + r;
+          float v[4];
+#pragma unroll
+          for (int c = 0; c < 4; c++) {
+            const int t = 4 * M + c;
+            v[c] = (jr < n && t < n) ? h_init(pc, psc, jr, t) : 0.0f;
+          }
+          H4[f] = make_float4(v[0], v[1], v[2], v[3]);
         }
       } else {
         const int ldh = (n + 3) & ~3;  // rows padded to 16 bytes; padding columns stay zero
@@ -1109,6 +1157,181 @@ __global__ void __launch_bounds__(NT, (HM == 1 && NT <= 256) ? 3 : 1) k_minimize
               }
             }
             __syncthreads();
+        } else if (SYM) {
+          // Symmetric blocked storage (sym_slot_floats): each kept 4x4 block B of the quad pair
+          // (L, M) is read once and serves both halves of H v: rows 4L.. gain B v_M and, off
+          // the diagonal, rows 4M.. gain B^T v_L. Wave w takes the distances s = w, w + NW, ...
+          // and its lanes walk (distance, L) items: for nq <= 64, 64/nq groups of nq lanes take
+          // one distance each; for nq >= 64 + NW the items run on consecutively, so that one
+          // instruction spans at most two distances NW apart, whose rows 4M.. are disjoint;
+          // in between, 64 lanes and then nq - 64 take each distance. The products go into the
+          // wave's private LDS accumulators (one copy per lane group, a float4 per quad) by a
+          // read, add and write of the quad: no two lanes of one instruction touch the same
+          // quad and a wave's LDS operations complete in order, so each sum is formed in a
+          // fixed order and results are deterministic. (LDS float atomics give the same sums
+          // but ran the medium workload at 302.6 ms against 119.0 ms: waves waited on LDS for
+          // 42% of their cycles.) When all NPM update slots are pending they are added to each
+          // element as it is read, oldest first, and the block is written back; otherwise
+          // the pending updates enter H v through their factored form, from dot products
+          // taken after the pass.
+          const int nq = (n + 3) >> 2, D = sym_dists(nq), n4 = nq << 2;
+          const int lane = tid & 63, w = tid >> 6;
+          const int Jw = w < D ? (D - 1 - w) / NW + 1 : 0;  // distances of this wave
+          const int G = nq <= 64 ? 64 / nq : 1;             // lane groups (accumulator copies)
+          float* acc = big + (size_t)w * sacc_w;
+          int j, L, cp = 0;
+          if (nq <= 64) {
+            cp = lane / nq;
+            L = lane - cp * nq;
+            j = cp < G ? cp : Jw;  // lanes past the last group stay idle
+          } else {
+            j = 0;
+            L = lane;
+          }
+          const int steps = nq <= 64 ? (Jw + G - 1) / G : nq < 64 + NW ? 2 * Jw : (Jw * nq + 63) >> 6;
+          const bool flush = npend == NPM;
+          const float4* VG4 = (const float4*)VG;  // VG4[i] = {VG[2i], VG[2i + 1]}
+          float4* H4 = (float4*)H;
+          for (int kk = 0; kk < steps; kk++) {
+            if (j < Jw && L < nq) {
+              const int s = w + j * NW;
+              const int M = L + s < nq ? L + s : L + s - nq;
+              if (!((nq & 1) == 0 && s == D - 1 && s != 0 && L >= (nq >> 1))) {
+                float4* bp = H4 + (size_t)(4 * s) * nq + L;
+                float4 b0 = bp[0], b1 = bp[nq], b2 = bp[2 * nq], b3 = bp[3 * nq];
+                if (flush) {
+#pragma unroll
+                  for (int k = 0; k < NPM; k++) {
+                    const float4* sk = (const float4*)(sh_slot(k) + 4 * L);  // {s, h} of rows 4L..
+                    const float4* ak = (const float4*)(ab_slot(k) + 4 * M);  // {alpha, beta} of columns 4M..
+                    const float4 s01 = sk[0], s23 = sk[1], a01 = ak[0], a23 = ak[1];
+                    const float ux[4] = {a01.x, a01.z, a23.x, a23.z}, uy[4] = {a01.y, a01.w, a23.y, a23.w};
+                    hquad_upd(b0, make_float2(s01.x, s01.y), ux, uy);
+                    hquad_upd(b1, make_float2(s01.z, s01.w), ux, uy);
+                    hquad_upd(b2, make_float2(s23.x, s23.y), ux, uy);
+                    hquad_upd(b3, make_float2(s23.z, s23.w), ux, uy);
+                  }
+                  bp[0] = b0;
+                  bp[nq] = b1;
+                  bp[2 * nq] = b2;
+                  bp[3 * nq] = b3;
+                }
+                // rows 4L.. gain B v_M; then, off the diagonal, rows 4M.. gain B^T v_L. Another
+                // lane of this wave may touch quad M in the first update or quad L in the second,
+                // so the compiler barriers keep each read behind the previous write (the
+                // hardware keeps a wave's LDS operations in order). (Keeping the first update in
+                // registers while a lane's L stays fixed measured no faster: 119.9 against
+                // 118.9 ms on the medium workload.)
+                const float4 vm01 = VG4[2 * M], vm23 = VG4[2 * M + 1];
+                float4* a4 = (float4*)(acc + cp * n4);
+                {
+                  float4 v = a4[L];
+                  v.x += fmaf(b0.w, vm23.w, fmaf(b0.z, vm23.y, fmaf(b0.y, vm01.w, b0.x * vm01.y)));
+                  v.y += fmaf(b1.w, vm23.w, fmaf(b1.z, vm23.y, fmaf(b1.y, vm01.w, b1.x * vm01.y)));
+                  v.z += fmaf(b2.w, vm23.w, fmaf(b2.z, vm23.y, fmaf(b2.y, vm01.w, b2.x * vm01.y)));
+                  v.w += fmaf(b3.w, vm23.w, fmaf(b3.z, vm23.y, fmaf(b3.y, vm01.w, b3.x * vm01.y)));
+                  a4[L] = v;
+                }
+                asm volatile("" ::: "memory");
+                if (s != 0) {
+                  const float4 vl01 = VG4[2 * L], vl23 = VG4[2 * L + 1];
+                  float4 v = a4[M];
+                  v.x += fmaf(b3.x, vl23.w, fmaf(b2.x, vl23.y, fmaf(b1.x, vl01.w, b0.x * vl01.y)));
+                  v.y += fmaf(b3.y, vl23.w, fmaf(b2.y, vl23.y, fmaf(b1.y, vl01.w, b0.y * vl01.y)));
+                  v.z += fmaf(b3.z, vl23.w, fmaf(b2.z, vl23.y, fmaf(b1.z, vl01.w, b0.z * vl01.y)));
+                  v.w += fmaf(b3.w, vl23.w, fmaf(b2.w, vl23.y, fmaf(b1.w, vl01.w, b0.w * vl01.y)));
+                  a4[M] = v;
+                }
+                asm volatile("" ::: "memory");
+              }
+            }
+            if (nq <= 64) {
+              j += G;
+            } else if (nq < 64 + NW) {
+              if (L < 64) {
+                L += 64;
+              } else {
+                L -= 64;
+                j++;
+              }
+            } else {
+              L += 64;
+              if (L >= nq) {
+                L -= nq;
+                j++;
+              }
+            }
+          }
+          // the pending updates' part of H g when they were not added element by element:
+          // y_t += alpha_t (s.g) + beta_t (h.g), with the dot products reduced in FP64
+          const bool fact = npend > 0 && npend < NPM;
+          float fs[NPM], fh[NPM];
+          if (fact) {
+            double ds[NPM], dh[NPM];
+#pragma unroll
+            for (int k = 0; k < NPM; k++) ds[k] = dh[k] = 0.0;
+            for (int i = tid; i < n; i += NT) {
+              const double g = VG[i].y;
+#pragma unroll
+              for (int k = 0; k < NPM; k++) {
+                if (k < npend) {
+                  const float2 u = sh_slot(k)[i];
+                  ds[k] += (double)u.x * g;
+                  dh[k] += (double)u.y * g;
+                }
+              }
+            }
+            if constexpr (2 * NPM <= 8) {
+              double v[2 * NPM];
+#pragma unroll
+              for (int k = 0; k < NPM; k++) {
+                v[k] = ds[k];
+                v[NPM + k] = dh[k];
+              }
+              R.template run<2 * NPM, 0>(v);
+#pragma unroll
+              for (int k = 0; k < NPM; k++) {
+                ds[k] = v[k];
+                dh[k] = v[NPM + k];
+              }
+            } else {
+              R.template run<NPM, 0>(ds);
+              R.template run<NPM, 0>(dh);
+            }
+#pragma unroll
+            for (int k = 0; k < NPM; k++) {
+              fs[k] = (float)ds[k];
+              fh[k] = (float)dh[k];
+            }
+          } else {
+            __syncthreads();  // every wave's accumulators are complete
+          }
+          // sum the waves' accumulators in a fixed order and clear them for the next pass
+          for (int t = tid; t < n4; t += NT) {
+            const int idx = t;
+            float y = 0.f;
+            for (int ww = 0; ww < NW; ww++)
+              for (int c = 0; c < G; c++) {
+                float* a = big + (size_t)ww * sacc_w + c * n4 + idx;
+                y += *a;
+                *a = 0.f;
+              }
+            if (t < n) {
+              if (fact) {
+#pragma unroll
+                for (int k = 0; k < NPM; k++)
+                  if (k < npend) {
+                    const float2 ab = ab_slot(k)[t];
+                    y = fmaf(ab.y, fh[k], fmaf(ab.x, fs[k], y));
+                  }
+              }
+              // H dg = H g_new - H g_old = H g_new + d_old
+              hd[t] = y + d[t];
+              d[t] = y;
+            }
+          }
+          __syncthreads();
+          if (npend == NPM) npend = 0;  // every pending update is in H now
         } else {
           // full storage in HBM with rows padded to ldh = round4(n) floats (16-byte aligned).
           // 2^lq consecutive lanes share the column quad 4tq..4tq+3 and split its rows into
@@ -1750,7 +1973,7 @@ int round4(int v) { return (v + 3) & ~3; }
 
 struct NkPlan {
   int nt = 256;
-  int hm = 0;  // Hessian placement of k_minimize_nk (0 packed LDS, 1 full HBM, 2 packed HBM)
+  int hm = 0;  // Hessian placement of k_minimize_nk (0 packed LDS, 1 full HBM, 2 packed HBM, 3 symmetric HBM)
   int tl = 0;  // term tables: 0 global, 1 all staged in LDS, 2 bonded staged
   int tbytes = 0;
   size_t lds = 0;
@@ -1772,6 +1995,8 @@ const void* nk_kernel_ptr() {
   X(0, 1, 0, 640) X(0, 1, 0, 768) X(0, 1, 0, 1024)                                                     \
   X(0, 1, 2, 64) X(0, 1, 2, 128) X(0, 1, 2, 192) X(0, 1, 2, 256) X(0, 1, 2, 320) X(0, 1, 2, 512)       \
   X(0, 2, 0, 256) X(0, 2, 0, 512) X(0, 2, 0, 640) X(0, 2, 0, 1024)                                     \
+  X(0, 3, 0, 64) X(0, 3, 0, 128) X(0, 3, 0, 256) X(0, 3, 0, 512)                                        \
+  X(0, 3, 2, 64) X(0, 3, 2, 128) X(0, 3, 2, 256) X(0, 3, 2, 512)                                        \
   X(1, 0, 1, 64) X(1, 0, 1, 128) X(1, 0, 1, 256)                                                       \
   X(1, 0, 0, 128) X(1, 0, 0, 256) X(1, 0, 0, 512)                                                      \
   X(1, 1, 0, 128) X(1, 1, 0, 192) X(1, 1, 0, 256) X(1, 1, 0, 512) X(1, 1, 2, 256)
@@ -1805,6 +2030,10 @@ int make_nk_plan(const mmffhip_batch* b, const mmffhip_nk_opts* o, NkPlan& pl) {
     pl.hm = 1;
   } else if (o->hess_place == MMFFHIP_HESS_GLOBAL_PACKED) {
     pl.hm = o->algo == MMFFHIP_BFGS ? 2 : 1;
+  } else if (o->hess_place == MMFFHIP_HESS_GLOBAL_SYM) {
+    // one triangle of blocks stands for both, so a learned start (whose 3x3 blocks need not
+    // be symmetric) keeps the full matrix
+    pl.hm = o->algo == MMFFHIP_BFGS && !o->precond ? 3 : 1;
   } else if (o->hess_place == MMFFHIP_HESS_AUTO) {
     pl.hm = lds_in <= (size_t)di.lds ? 0 : 1;
   } else {
@@ -1845,6 +2074,7 @@ int make_nk_plan(const mmffhip_batch* b, const mmffhip_nk_opts* o, NkPlan& pl) {
   if (pl.hm != 0) {
     pl.slot_floats = o->algo == MMFFHIP_LBFGS ? (size_t)2 * m * pl.max_n
                      : pl.hm == 2             ? packed_floats(pl.max_n)
+                     : pl.hm == 3             ? sym_slot_floats(pl.max_n)
                                               : (size_t)pl.max_n * pl.max_n;
     pl.slot_floats = (pl.slot_floats + 63) & ~(size_t)63;
   }
@@ -2005,8 +2235,11 @@ int mmffhip_minimize_nk_plan(const mmffhip_batch* b, const mmffhip_nk_opts* o, i
   *grid = b->n_confs == 0 ? 0 : pl.grid;
   if (block) *block = pl.nt;
   if (lds_bytes) *lds_bytes = (int32_t)pl.lds;
-  if (hess_place) *hess_place = pl.hm == 0 ? MMFFHIP_HESS_LDS : pl.hm == 2 ? MMFFHIP_HESS_GLOBAL_PACKED
-                                                                           : MMFFHIP_HESS_GLOBAL;
+  if (hess_place)
+    *hess_place = pl.hm == 0   ? MMFFHIP_HESS_LDS
+                  : pl.hm == 2 ? MMFFHIP_HESS_GLOBAL_PACKED
+                  : pl.hm == 3 ? MMFFHIP_HESS_GLOBAL_SYM
+                               : MMFFHIP_HESS_GLOBAL;
   return MMFFHIP_OK;
 }
 

@@ -100,6 +100,16 @@ def resident_workgroups(desc, o) -> int:
     return int(g.value)
 
 
+def planned_placement(desc, o) -> int:
+    """The Hessian placement the minimize_nk launch will use (HESS_LDS / _GLOBAL /
+    _GLOBAL_PACKED / _GLOBAL_SYM): AUTO resolved, and a symmetric or packed request with a
+    learned start or L-BFGS mapped to the full matrix, as the planner does."""
+    L = _lib.load()
+    g, hp = C.c_int32(), C.c_int32()
+    _lib.check(L.mmffhip_minimize_nk_plan(C.byref(desc), C.byref(o), C.byref(g), None, None, C.byref(hp)))
+    return int(hp.value)
+
+
 def table_lds_bytes(hb: HostBatch, pairs: bool = True) -> np.ndarray:
     """Per-molecule bytes of the LDS staging layout of its term tables (csrc table_lds_bytes);
     without ``pairs``, of its bonded tables only (what a global-Hessian launch stages)."""
@@ -111,22 +121,23 @@ def table_lds_bytes(hb: HostBatch, pairs: bool = True) -> np.ndarray:
 
 # Size buckets for the nk minimizer: (max 3N, threads per workgroup, Hessian placement).
 # Each bucket is its own persistent launch sized for its largest molecule. Every default
-# bucket keeps a full-storage inverse Hessian in a global-memory slot per resident workgroup
-# (L2 / Infinity Cache resident, read as coalesced 16-byte column quads): those workgroups
-# need little LDS, so registers set the residency (three waves per SIMD up to 256 threads)
-# and one conformer's memory waits overlap the others' term evaluation. Small molecules run
-# one conformer per wave (64 threads, 12 conformers per CU), where a packed LDS triangle
-# holds 4-7 conformers per CU. Above n = 360 one 512-thread workgroup per CU does better:
-# three large Hessians per CU outgrow the Infinity Cache. Measured on MI355X, whole size
-# classes of the ragged workload (200k conformers), ms per class:
-#   n <= 96:        64 threads 56.5, 128 threads 66.2, LDS triangle at 128 threads 90.9
-#   96 < n <= 144:  64: 126.2, 128: 125.7, 192: 147.6, 256: 168.5
-#   144 < n <= 192: 64: 242.1, 128: 199.2, 192: 196.9, 256: 226.0
-#   192 < n <= 264: 128: 591.2, 192: 548.4, 256: 509.6
-#   264 < n <= 360: 128: 1841.5, 192: 1615.4, 256: 1487.7, 512: 1777.7
-#   n > 360:        256: 1986.3, 512: 1727.8
-DEFAULT_BUCKETS = ((96, 64, _lib.HESS_GLOBAL), (192, 128, _lib.HESS_GLOBAL), (360, 256, _lib.HESS_GLOBAL),
-                   (1 << 20, 512, _lib.HESS_GLOBAL))
+# bucket keeps the inverse Hessian in a global-memory slot per resident workgroup (L2 /
+# Infinity Cache resident) as symmetric 4x4 blocks, one per pair of row quads: half the bytes
+# of the full matrix, each block read once per iteration for both halves of H v. Those
+# workgroups need little LDS, so registers set the residency (three waves per SIMD up to 256
+# threads) and one conformer's memory waits overlap the others' term evaluation. Small
+# molecules run one conformer per wave (64 threads, 12 conformers per CU). Measured on MI355X
+# by autotune_buckets on 8,192-conformer samples of each class of the ragged workload, ms
+# (sym = symmetric blocks, global = full matrix, lds = packed triangle in LDS):
+#   n <= 96:        64 sym 10.6, 64 global 11.7, 128 global 12.3, 128 lds 16.1, 64 lds 21.5
+#   96 < n <= 192:  128 sym 39.3, 128 global 41.9, 192 global 43.7, 256 global 47.8, 64 sym 47.9
+#   192 < n <= 264: 256 sym 122.5, 128 sym 125.8, 256 global 136.8, 192 global 159.9
+#   264 < n <= 360: 256 sym 256.4, 512 sym 405.6, 256 global 429.3, 192 global 452.0, 512 global 474.8
+#   n > 360:        256 sym 514.9, 512 sym 698.7, 512 global 846.1, 640 global 942.9, 256 global 1006.2
+# With the full matrix, one 512-thread workgroup per CU did best above n = 360 (three large
+# Hessians per CU outgrew the Infinity Cache); at half the bytes, 256 threads do.
+_SYM = _lib.HESS_GLOBAL_SYM
+DEFAULT_BUCKETS = ((96, 64, _SYM), (192, 128, _SYM), (360, 256, _SYM), (1 << 20, 256, _SYM))
 # L-BFGS keeps its (s, y) history in global memory: 128 threads up to n = 600, 256 above.
 # Whole classes, m = 10, maxIters 2000, ms per class: ragged n <= 192 (119,560 conformers)
 # 704 with global history against 1233 with the history in LDS; ragged 192 < n <= 600
@@ -196,7 +207,7 @@ class NkRunner:
             desc.n_confs = len(sel)
             desc.max_atoms = int(hb.n_atoms[hb.conf_mol[sel]].max())
             hp = user_place or place
-            per_mol = bbytes_mol if hp == _lib.HESS_GLOBAL else tbytes_mol
+            per_mol = bbytes_mol if hp in (_lib.HESS_GLOBAL, _lib.HESS_GLOBAL_SYM) else tbytes_mol
             tb = int(per_mol[hb.conf_mol[sel]].max()) if stage else 0
             bopts = dict(opts, use_lbfgs=algo == "lbfgs")
             o = nk_options(block=user_block or block, hess_place=hp, table_lds_bytes=tb, **bopts)
@@ -215,7 +226,8 @@ class NkRunner:
             ws = torch.empty(int(nbytes.value), dtype=torch.uint8, device=dev)
             ordt = torch.from_numpy(np.ascontiguousarray(sel)).to(dev)
             self.launches.append(dict(desc=desc, opts=o, ws=ws, ws_bytes=int(nbytes.value), order=ordt,
-                                      n=len(sel), max_n=3 * desc.max_atoms, tbytes=tb, algo=algo, xcd_off=xoff))
+                                      n=len(sel), max_n=3 * desc.max_atoms, tbytes=tb, algo=algo, xcd_off=xoff,
+                                      placement=planned_placement(desc, o)))
         assert assigned[order].all(), "a conformer fits no bucket"
         C_ = db.n_confs
         self.e = torch.empty(C_, dtype=torch.float64, device=dev)
@@ -286,12 +298,13 @@ class NkRunner:
 # candidate (block, placement) pairs per size class for the autotuner; LDS candidates
 # whose plan does not fit are skipped
 TUNE_CLASSES = (
-    (96, ((64, _lib.HESS_GLOBAL), (128, _lib.HESS_GLOBAL), (64, _lib.HESS_LDS), (128, _lib.HESS_LDS))),
+    (96, ((64, _lib.HESS_GLOBAL), (128, _lib.HESS_GLOBAL), (64, _lib.HESS_LDS), (128, _lib.HESS_LDS), (64, _SYM))),
     (192, ((64, _lib.HESS_GLOBAL), (128, _lib.HESS_GLOBAL), (192, _lib.HESS_GLOBAL), (256, _lib.HESS_GLOBAL),
-           (192, _lib.HESS_LDS))),
-    (264, ((128, _lib.HESS_GLOBAL), (192, _lib.HESS_GLOBAL), (256, _lib.HESS_GLOBAL), (640, _lib.HESS_LDS))),
-    (360, ((192, _lib.HESS_GLOBAL), (256, _lib.HESS_GLOBAL), (512, _lib.HESS_GLOBAL))),
-    (1 << 20, ((256, _lib.HESS_GLOBAL), (512, _lib.HESS_GLOBAL), (640, _lib.HESS_GLOBAL))),
+           (192, _lib.HESS_LDS), (64, _SYM), (128, _SYM))),
+    (264, ((128, _lib.HESS_GLOBAL), (192, _lib.HESS_GLOBAL), (256, _lib.HESS_GLOBAL), (640, _lib.HESS_LDS),
+           (128, _SYM), (256, _SYM))),
+    (360, ((192, _lib.HESS_GLOBAL), (256, _lib.HESS_GLOBAL), (512, _lib.HESS_GLOBAL), (256, _SYM), (512, _SYM))),
+    (1 << 20, ((256, _lib.HESS_GLOBAL), (512, _lib.HESS_GLOBAL), (640, _lib.HESS_GLOBAL), (256, _SYM), (512, _SYM))),
 )
 
 

@@ -137,7 +137,7 @@ def test_bucket_tables_cover_every_size_and_name_known_placements():
     autotuner candidate names a block the planner accepts and a known placement. The staged
     bonded-table size is the full staging size without the pair tables."""
     from mlxmolkit_archive_amd import _lib, engine
-    places = {_lib.HESS_AUTO, _lib.HESS_LDS, _lib.HESS_GLOBAL, _lib.HESS_GLOBAL_PACKED}
+    places = {_lib.HESS_AUTO, _lib.HESS_LDS, _lib.HESS_GLOBAL, _lib.HESS_GLOBAL_PACKED, _lib.HESS_GLOBAL_SYM}
     for table in (engine.DEFAULT_BUCKETS, engine.LBFGS_BUCKETS):
         bounds = [b for b, _, _ in table]
         assert bounds == sorted(bounds) and bounds[-1] >= 3 * 65535
