@@ -42,8 +42,8 @@ enum mmffhip_algo { MMFFHIP_BFGS = 0, MMFFHIP_LBFGS = 1 };
 /* where k_minimize_nk keeps the dense inverse Hessian (L-BFGS: its history): AUTO = LDS when it
  * fits, else HBM; LDS = packed upper triangle in LDS; GLOBAL = full matrix in HBM; GLOBAL_PACKED
  * = packed upper triangle in HBM (dense BFGS; L-BFGS treats it as GLOBAL); GLOBAL_SYM = one
- * 4x4 block per pair of row quads in HBM, half the bytes of GLOBAL (dense BFGS without a
- * learned start; otherwise treated as GLOBAL) */
+ * 4x4 block per pair of row quads in HBM, half the bytes of GLOBAL (dense BFGS; L-BFGS treats
+ * it as GLOBAL) */
 enum mmffhip_hess_place {
   MMFFHIP_HESS_AUTO = 0,
   MMFFHIP_HESS_LDS = 1,
@@ -101,9 +101,9 @@ typedef struct {
                           gradient pass when the first trial is accepted) */
   const float *precond;  /* optional per-atom 3x3 initial inverse-Hessian blocks, 9 floats per
                             atom at (conf_off/3 + atom)*9, in units of (0.1 * Hessian block)^-1
-                            (the kernel multiplies them by 0.1 / initial gradient scale); BFGS
-                            restarts from the identity if a line search fails. NULL = identity
-                            (reference) */
+                            (the kernel multiplies them by 0.1 / initial gradient scale and
+                            starts from their symmetric part (B + B^T) / 2); BFGS restarts from
+                            the identity if a line search fails. NULL = identity (reference) */
   int32_t table_lds_bytes; /* >0: stage each molecule's term tables in LDS, reserving this many
                               bytes: all tables beside an LDS Hessian; beside a global Hessian
                               only the bonded tables (pairs stay in global memory), and only

@@ -767,11 +767,17 @@ __host__ __device__ inline size_t nk_lds_bytes(int algo, int hm, int nt, int max
   return b;
 }
 
-// initial inverse Hessian element: the identity, or the learned per-atom 3x3 block times
-// psc (blocks approximate (0.1 * Hessian block)^-1; psc = 0.1 / gradient scale rescales them
-// to the scaled gradient the minimizer works with)
+// initial inverse Hessian element: the identity, or the symmetric part of the learned
+// per-atom 3x3 block times psc (blocks approximate (0.1 * Hessian block)^-1; psc = 0.1 /
+// gradient scale rescales them to the scaled gradient the minimizer works with). BFGS keeps
+// H symmetric, and every placement reads one orientation of each element, so the start is
+// made symmetric rather than taken from one triangle of the block.
 __device__ __forceinline__ float h_init(const float* pc, float psc, int i, int j) {
-  if (pc) return (i / 3) == (j / 3) ? psc * pc[(j / 3) * 9 + (i % 3) * 3 + (j % 3)] : 0.0f;
+  if (pc) {
+    if ((i / 3) != (j / 3)) return 0.0f;
+    const float* b = pc + (j / 3) * 9;
+    return psc * (0.5f * (b[(i % 3) * 3 + (j % 3)] + b[(j % 3) * 3 + (i % 3)]));
+  }
   return i == j ? 1.0f : 0.0f;
 }
 
@@ -1221,7 +1227,8 @@ __global__ void __launch_bounds__(NT, ((HM == 1 || HM == 3) && NT <= 256) ? 3 : 
                 // so the compiler barriers keep each read behind the previous write (the
                 // hardware keeps a wave's LDS operations in order). (Keeping the first update in
                 // registers while a lane's L stays fixed measured no faster: 119.9 against
-                // 118.9 ms on the medium workload.)
+                // 118.9 ms on the medium workload; loading the next block's rows while this one
+                // is used spilled 76-244 bytes per lane and measured 125.1 against 119.5 ms.)
                 const float4 vm01 = VG4[2 * M], vm23 = VG4[2 * M + 1];
                 float4* a4 = (float4*)(acc + cp * n4);
                 {
@@ -2031,9 +2038,7 @@ int make_nk_plan(const mmffhip_batch* b, const mmffhip_nk_opts* o, NkPlan& pl) {
   } else if (o->hess_place == MMFFHIP_HESS_GLOBAL_PACKED) {
     pl.hm = o->algo == MMFFHIP_BFGS ? 2 : 1;
   } else if (o->hess_place == MMFFHIP_HESS_GLOBAL_SYM) {
-    // one triangle of blocks stands for both, so a learned start (whose 3x3 blocks need not
-    // be symmetric) keeps the full matrix
-    pl.hm = o->algo == MMFFHIP_BFGS && !o->precond ? 3 : 1;
+    pl.hm = o->algo == MMFFHIP_BFGS ? 3 : 1;
   } else if (o->hess_place == MMFFHIP_HESS_AUTO) {
     pl.hm = lds_in <= (size_t)di.lds ? 0 : 1;
   } else {

@@ -698,8 +698,8 @@ typedef struct {
 } lbfgs_hist;
 
 /* Dense inverse Hessian start: the identity, or (h0 != NULL) the block-diagonal matrix of the
- * per-atom 3x3 blocks h0 scaled by GRAD_SCALE_INIT / gs -- the learned preconditioner of the
- * HIP minimizer, which has no counterpart in the reference. */
+ * symmetric parts (B + B^T) / 2 of the per-atom 3x3 blocks h0, scaled by GRAD_SCALE_INIT / gs --
+ * the learned preconditioner of the HIP minimizer, which has no counterpart in the reference. */
 static void init_hessian(double *H, int n, const double *h0, double psc) {
     memset(H, 0, sizeof(double) * (size_t)n * n);
     if (!h0) {
@@ -708,7 +708,8 @@ static void init_hessian(double *H, int n, const double *h0, double psc) {
     }
     for (int a = 0; a < n / 3; a++)
         for (int r = 0; r < 3; r++)
-            for (int c = 0; c < 3; c++) H[(size_t)(3 * a + r) * n + 3 * a + c] = psc * h0[9 * a + 3 * r + c];
+            for (int c = 0; c < 3; c++)
+                H[(size_t)(3 * a + r) * n + 3 * a + c] = psc * 0.5 * (h0[9 * a + 3 * r + c] + h0[9 * a + 3 * c + r]);
 }
 
 static int nk_minimize(const oracle_mol *mol, double *x, int max_iters, double grad_tol, int diel,
