@@ -156,8 +156,8 @@ int mmffhip_minimize_nk(const mmffhip_batch *b, const mmffhip_nk_opts *o, const 
  * inputs and outputs. No counterpart in the reference: it is for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc). The persistent grid is capped to the slots that fit; a plan that needs
- * more than 64 MiB for one slot per compute unit (a full global Hessian above n = 3N = 256 on
- * 256 CUs) fails with MMFFHIP_ERR_SIZE. Results are identical to mmffhip_minimize_nk's.
+ * more than 64 MiB for one slot per compute unit (on 256 CUs: a full global Hessian above
+ * n = 3N = 256, symmetric blocks above n = 356) fails with MMFFHIP_ERR_SIZE. Results are identical to mmffhip_minimize_nk's.
  * Every call on one GPU shares that GPU's array, which keeps the last call's contents: calls
  * that overlap on one GPU (concurrent streams or threads) race and must be serialised by the
  * caller. */
