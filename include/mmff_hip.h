@@ -39,8 +39,9 @@ enum mmffhip_status {
 
 enum mmffhip_family { MMFFHIP_FAMILY_TG = 0, MMFFHIP_FAMILY_MEGA = 1 };
 enum mmffhip_algo { MMFFHIP_BFGS = 0, MMFFHIP_LBFGS = 1 };
-/* where k_minimize_nk keeps the dense inverse Hessian (L-BFGS: its history): AUTO = LDS when it
- * fits, else HBM; LDS = packed upper triangle in LDS; GLOBAL = full matrix in HBM; GLOBAL_PACKED
+/* where k_minimize_nk keeps the dense inverse Hessian (L-BFGS: its history): AUTO = GLOBAL_SYM
+ * for dense BFGS, and for L-BFGS the history in LDS when it fits, else in HBM; LDS = packed
+ * upper triangle in LDS; GLOBAL = full matrix in HBM; GLOBAL_PACKED
  * = packed upper triangle in HBM (dense BFGS; L-BFGS treats it as GLOBAL); GLOBAL_SYM = one
  * 4x4 block per pair of row quads in HBM, half the bytes of GLOBAL (dense BFGS; L-BFGS treats
  * it as GLOBAL) */

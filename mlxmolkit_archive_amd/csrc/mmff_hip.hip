@@ -2,9 +2,10 @@
 //
 // Execution model (one conformer per workgroup, persistent grid with a work queue):
 //   * positions, gradient accumulators and BFGS/L-BFGS vectors live in LDS;
-//   * the dense inverse Hessian lives in LDS as a packed triangle with 16-byte aligned
-//     columns when it fits (n = 3N up to 264 at one workgroup per CU), otherwise in full
-//     storage in a per-workgroup global slot, read as coalesced 16-byte column quads;
+//   * the dense inverse Hessian lives in a per-workgroup global slot as symmetric 4x4 blocks,
+//     one per pair of row quads, each read once per iteration for both halves of H v (the
+//     default), or in full storage read as coalesced 16-byte column quads, or in LDS as a
+//     packed triangle with 16-byte aligned columns (n = 3N up to 264 at one workgroup per CU);
 //   * term energies and gradients are FP32; each lane's energy is accumulated in FP64 and
 //     reduced with wave shuffles + one LDS exchange, so the Armijo test and reported
 //     energies are FP64 sums; gradients are summed exactly in 64-bit fixed point with LDS
@@ -2040,7 +2041,9 @@ int make_nk_plan(const mmffhip_batch* b, const mmffhip_nk_opts* o, NkPlan& pl) {
   } else if (o->hess_place == MMFFHIP_HESS_GLOBAL_SYM) {
     pl.hm = o->algo == MMFFHIP_BFGS ? 3 : 1;
   } else if (o->hess_place == MMFFHIP_HESS_AUTO) {
-    pl.hm = lds_in <= (size_t)di.lds ? 0 : 1;
+    // dense BFGS: the symmetric blocks in HBM (they beat the LDS triangle even where it
+    // fits); L-BFGS: the history in LDS when it fits, else in HBM
+    pl.hm = o->algo == MMFFHIP_BFGS ? 3 : lds_in <= (size_t)di.lds ? 0 : 1;
   } else {
     return fail(MMFFHIP_ERR_ARG, "unknown hess_place");
   }

@@ -236,8 +236,9 @@ class NkRunner:
 
     def run(self, x_in, x_out=None, stream=None, overlap: bool = False, events=None) -> NkResult:
         """Minimize every conformer of the plan. The size-bucket launches run one after another
-        on ``stream`` (ragged workload on MI355X: 4.29 s per step, against 4.49 s with the
-        launches overlapped, when the buckets' Hessians compete for the Infinity Cache). With
+        on ``stream`` (ragged workload on MI355X: 2.66 s per step, against 2.68-2.74 s with the
+        launches overlapped; 4.29 against 4.49 s with the full-matrix Hessians, which competed
+        for the Infinity Cache). With
         ``overlap`` they go to up to three side streams instead (forked from and joined back to
         ``stream``); the launches write disjoint conformers, so results are the same.
         ``events``: optional list of (start, end) torch.cuda.Event pairs, one per launch,

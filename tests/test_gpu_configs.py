@@ -217,6 +217,11 @@ def test_plan_query_and_bucket_folding(ragged_sample):
                                               C.byref(hp)))
         assert g.value == min(dbs.n_confs, res) and b.value == block and hp.value == G
         assert 0 < lds.value <= 160 * 1024
+    # AUTO resolves to the symmetric blocks for dense BFGS
+    o = engine.nk_options(max_iters=10, block=128)
+    g, hp = C.c_int32(), C.c_int32()
+    _lib.check(L.mmffhip_minimize_nk_plan(C.byref(dbs.desc), C.byref(o), C.byref(g), None, None, C.byref(hp)))
+    assert hp.value == _lib.HESS_GLOBAL_SYM
     db = engine.device_batch(wl.params, wl.conf_counts)
     r = engine.NkRunner(db, max_iters=10, grad_tol=1e-4)
     assert [(l["opts"].block, l["n"]) for l in r.launches] == [(256, db.n_confs)]

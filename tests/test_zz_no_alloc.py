@@ -1,8 +1,9 @@
 """mmffhip_minimize_nk_noalloc (scratch in a library-owned device array, no workspace) against
 mmffhip_minimize_nk (caller workspace), called through the C ABI: identical positions,
-energies, statuses and counters on two calls in a row, the second with a smaller batch, for a
-global-Hessian plan (the scratch holds the per-workgroup slots, and the grid is capped to the
-slots that fit) and for an LDS plan (the scratch holds only the work queue). A plan that needs
+energies, statuses and counters on two calls in a row, the second with a smaller batch, for
+global-Hessian plans (full matrix and symmetric blocks: the scratch holds the per-workgroup
+slots, and the grid is capped to the slots that fit) and for an LDS plan (the scratch holds
+only the work queue). A plan that needs
 more than the scratch for one slot per compute unit is refused with a size error before
 anything is launched."""
 from __future__ import annotations
@@ -54,12 +55,14 @@ def _both(wl, place, block):
     return outs
 
 
-@pytest.mark.parametrize("place,block", [(2, 256), (1, 128)])
+@pytest.mark.parametrize("place,block", [(2, 256), (1, 128), (4, 256)])
 def test_noalloc_matches_workspace_entry_point_twice(place, block):
-    """Two calls in a row through both entry points, the second batch smaller. Global plan:
-    small-generator molecules (n <= 150): slots of 92 KB, of which the 64 MiB array holds
-    726, so the 750-conformer batch's grid (three 256-thread workgroups per CU) is capped to
-    726 workgroups. LDS plan: the same batches' molecules with n <= 96."""
+    """Two calls in a row through both entry points, the second batch smaller. Full-matrix
+    global plan: small-generator molecules (n <= 150): slots of 92 KB, of which the 64 MiB
+    array holds 726, so the 750-conformer batch's grid (three 256-thread workgroups per CU) is
+    capped to 726 workgroups. Symmetric-block global plan (the default placement): the same
+    batches, slots of 48 KB, the grid not capped. LDS plan: the same batches' molecules with
+    n <= 96."""
     from mlxmolkit_archive_amd import workloads
     from test_gpu_configs import _select
     big = workloads.build("small", n_mols=150, n_confs=5)
