@@ -437,3 +437,45 @@ def test_learned_blocks_on_medium_molecules_match_oracle():
     assert within >= 0.9 * ok.sum()
     assert np.median(np.abs((e_gpu - e_ref)[ok])) < 2e-3
     assert abs(cnt[ok, 0].mean() - c_ref[ok, 0].mean()) <= 0.1 * c_ref[ok, 0].mean()
+
+
+def test_symmetric_pass_tracks_full_matrix_in_every_lane_mapping():
+    """The symmetric blocks' H pass walks its blocks in one of three lane mappings, chosen by
+    nq = round4(n)/4 and the waves per workgroup NW: lane groups of one distance each
+    (nq <= 64), 64 lanes then nq - 64 per distance (64 < nq < 64 + NW), or items running on
+    across distances (nq >= 64 + NW). Molecules are picked so that each mapping is hit at 64,
+    128 and 256 threads, with even and odd nq. After 10 BFGS iterations (two write-backs of
+    four pending updates) the positions agree with the full-matrix pass to 1e-3 A and the
+    energies to 1e-3 relative: the two differ only in summation order."""
+    from mlxmolkit_archive_amd import engine, workloads, _lib
+    wl = workloads.build("ragged", n_mols=600, n_confs=1)
+    n3 = 3 * np.array([p.n_atoms for p in wl.params])
+    nq = (n3 + 3) // 4
+    pick = []
+    for lo, hi in ((1, 24), (25, 32), (33, 48), (49, 63), (64, 64), (65, 65), (66, 67), (68, 68), (69, 71),
+                   (72, 90), (91, 118)):
+        idx = np.nonzero((nq >= lo) & (nq <= hi))[0]
+        # an even and an odd nq where the range has both
+        for par in (0, 1):
+            sel = idx[nq[idx] % 2 == par]
+            if len(sel):
+                pick.append(int(sel[0]))
+    assert len(pick) >= 14, sorted(set(nq[pick].tolist()))
+    sub = _select(wl, pick)
+    db = engine.device_batch(sub.params, sub.conf_counts)
+    xd = torch.from_numpy(sub.x).cuda()
+    print(f"\nnq of the picked molecules: {sorted(nq[pick].tolist())}")
+    for block in (64, 128, 256):
+        out = {}
+        for hp in (_lib.HESS_GLOBAL, _lib.HESS_GLOBAL_SYM):
+            r = engine.NkRunner(db, buckets=((1 << 20, block, hp),), max_iters=10, grad_tol=1e-4, ls_grad=True)
+            res = r.run(xd)
+            torch.cuda.synchronize()
+            out[hp] = (res.x.cpu().numpy().copy(), res.energy.cpu().numpy().copy(), res.counters.cpu().numpy().copy())
+        (xa, ea, ca), (xb, eb, cb) = out[_lib.HESS_GLOBAL], out[_lib.HESS_GLOBAL_SYM]
+        dx = np.abs(xa - xb).max()
+        de = np.max(np.abs(ea - eb) / np.maximum(np.abs(ea), 1.0))
+        print(f"block {block}: max |dx| {dx:.2e} A, max relative dE {de:.2e}, "
+              f"iterations equal {np.mean(ca[:, 0] == cb[:, 0]):.2f}")
+        assert dx < 1e-3 and de < 1e-3
+        assert np.array_equal(ca[:, 0], cb[:, 0])
