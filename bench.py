@@ -91,6 +91,8 @@ def main():
     ap.add_argument("--hess-place", default="default", choices=["default", "global", "sym"],
                     help="where the dense-BFGS buckets keep the inverse Hessian: default = engine.DEFAULT_BUCKETS; "
                          "global = the full matrix in HBM; sym = one 4x4 block per pair of row quads in HBM")
+    ap.add_argument("--one-launch", action="store_true",
+                    help="every conformer in one persistent 256-thread launch (no size buckets; longest first)")
     ap.add_argument("--sample", type=int, default=2048, help="conformers checked by the FP64 oracle")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--oracle-restart", action="store_true",
@@ -162,6 +164,9 @@ def main():
         base = tuple((b, blk, hp) for b, blk, _ in base)
         if not args.lbfgs:
             buckets = base
+    if args.one_launch:
+        assert not args.lbfgs_above, "--one-launch and --lbfgs-above exclude each other"
+        buckets = ((1 << 20, 256, base[-1][2]),) if not args.lbfgs else ((1 << 20, 256, _lib.HESS_GLOBAL),)
     if args.lbfgs_above > 0:
         assert not args.lbfgs and pc is None, "--lbfgs-above mixes methods; it excludes --lbfgs and --precond"
         buckets = []
@@ -367,9 +372,11 @@ def main():
                                 + (" from learned H0" if pc is not None else ""), "max_iters": args.max_iters,
                    "grad_tol": args.grad_tol, "atoms_min_max": [int(wl.atoms_per_conf().min()),
                                                                 int(wl.atoms_per_conf().max())],
-                   "parallelism": "1 GPU, one workgroup per conformer, persistent size-bucketed launches, per-XCD work queues",
-                   "launch": "eager, one launch per size bucket, in sequence" if graph is None
-                             else "HIP graph replay"},
+                   "parallelism": "1 GPU, one workgroup per conformer, "
+                                  + ("one persistent launch" if args.one_launch else "persistent size-bucketed launches")
+                                  + ", per-XCD work queues",
+                   "launch": ("eager, one launch" if args.one_launch else "eager, one launch per size bucket, in sequence")
+                             if graph is None else "HIP graph replay"},
         "mean_iters_to_converge": round(mean_iters, 1),
         "status0_fraction": round(n_status0 / wl.n_confs, 4),
         "within_0.1_fraction_sample": None if frac is None else round(frac, 4),
