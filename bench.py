@@ -372,6 +372,7 @@ def main():
                                 + (" from learned H0" if pc is not None else ""), "max_iters": args.max_iters,
                    "grad_tol": args.grad_tol, "atoms_min_max": [int(wl.atoms_per_conf().min()),
                                                                 int(wl.atoms_per_conf().max())],
+                   "hessian": "/".join(dict.fromkeys(l["hessian"] for l in launches)),
                    "parallelism": "1 GPU, one workgroup per conformer, "
                                   + ("one persistent launch" if args.one_launch else "persistent size-bucketed launches")
                                   + ", per-XCD work queues",
